@@ -1,0 +1,242 @@
+"""Benchmark: periodic P_l(k) / xi_l(r) multipoles, l <= 4, on a 512^3 grid.
+
+Workload (one "correlation"): the interlaced autocorrelation of a
+device-resident density grid pair -- 8M Thomas-process points (80,000
+parents, Poisson(100) children, sigma 3 Mpc/h) in a periodic 2000 Mpc/h box,
+CIC assignment with and without a half-cell shift, delta = rho/mean - 1,
+sinc^4 window correction, k bins 0-0.5 h/Mpc width 0.005 (k subgrid 321^3),
+r bins 0-200 Mpc/h width 5 (r subgrid 105^3).  Each step runs both forward
+FFTs, the interlacing combine, |F|^2, the k-space multipole histograms, the
+inverse FFT and the r-space multipole histograms, and leaves FP64 histograms
+on the device.
+
+Reported: correlations/s of the FP32-mixed form (the headline `value`),
+the FP64 plain form beside it, their ratio, the FP32-vs-FP64 error per
+multipole, p50/p99 latency, the dominant kernel's HBM roofline, and the FP64
+CPU oracle (scipy pocketfft, all host threads) timed on one correlation.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+CFG = dict(n=512, box=2000.0, parents=80_000, children=100, sigma=3.0, seed=4,
+           rmax=200.0, dr=5.0, kmax=0.5, dk=0.005, maxell=4, window=3, scheme=3)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_grids(lib, ctx, pos, cfg, precision):
+    n = cfg["n"]
+    cell = cfg["box"] / n
+    grids = []
+    ma_ms = 0.0
+    for shift in (0.0, 0.5):
+        g = lib.DeviceGrid((n, n, n), (0, 0, 0), cell, precision, ctx)
+        st = lib.MaStats()
+        t0 = time.perf_counter()
+        lib.check(lib.lib().fc_mass_assign(g.handle, lib.dptr(pos), pos.shape[0], 3, None, 1.0,
+                                           cfg["scheme"], 1, shift, st))
+        ma_ms += (time.perf_counter() - t0) * 1e3
+        g.normalize(st.totw / float(n) ** 3)
+        grids.append(g)
+    return grids, ma_ms, st
+
+
+def stage_bytes(cfg, s_work, s_grid, ks, rs):
+    """Algorithmic HBM bytes per launch of each stage (complex half grid
+    n^2 (n/2+1) elements of 2 s_work bytes; real grid n^3 of s_grid)."""
+    n = cfg["n"]
+    nh = n // 2 + 1
+    half = n * n * nh * 2 * s_work
+    real = n ** 3
+    return {
+        "fft_z_fwd": real * s_grid + half,
+        "fft_y_fwd": 2 * half, "fft_x_fwd": 2 * half,
+        "fft_x_inv": 2 * half, "fft_y_inv": 2 * half,
+        "fft_z_inv": half + real * s_work,
+        "combine": 2.5 * half,  # interlace (read 2, write 1) and |F|^2 (read 1, write 1), averaged
+        "hist_k": ks * 2 * s_work,
+        "hist_r": rs * s_work,
+    }
+
+
+def run_form(lib, corr_mod, ctx, pos, cfg, precision, steps, warmup, profile):
+    n = cfg["n"]
+    cell = cfg["box"] / n
+    grids, ma_ms, st = make_grids(lib, ctx, pos, cfg, precision)
+    c = corr_mod.PeriodicCorrelator((n, n, n), cell, cfg["window"], cfg["rmax"], cfg["dr"],
+                                    cfg["kmax"], cfg["dk"], cfg["maxell"], precision=precision)
+    L = lib.lib()
+    h = c._handle
+    ga, gb = grids[0].handle, grids[1].handle
+    for _ in range(warmup):
+        lib.check(L.fc_corr_autocorrelate_interlaced(h, ga, gb))
+    ctx.synchronize()
+    if profile:
+        c.set_profiling(True)
+    lat = []
+    t = ctypes.c_double()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lib.check(L.fc_corr_autocorrelate_interlaced(h, ga, gb))
+        lib.check(L.fc_corr_last_time_ms(h, ctypes.byref(t)))
+        lat.append(t.value)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    stages = c.stage_times() if profile else None
+    c._fetch()
+    res = {k: np.array(v, copy=True) if isinstance(v, np.ndarray) else v
+           for k, v in c.results().items()}
+    out = dict(wall_s=wall, lat_ms=np.array(lat), stages=stages, res=res, ma_ms=ma_ms,
+               totw=st.totw, kshape=c.kshape, rshape=c.rshape)
+    deltas = None
+    return out, grids, c
+
+
+def floor_err(x, ref):
+    floor = 1e-3 * np.max(np.abs(ref[0]))
+    return np.abs(x - ref) / np.maximum(np.abs(ref), floor)
+
+
+def cpu_baseline(grids, cfg):
+    """FP64 oracle (scipy.fft with all host threads) on one interlaced
+    correlation of the same grids."""
+    from oracle import fftcorr_oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    O.FFT_WORKERS = threads
+    n = cfg["n"]
+    da = grids[0].download()
+    db = grids[1].download()
+    ref = O.PeriodicCorrelator((n, n, n), cfg["box"] / n, cfg["window"], cfg["rmax"], cfg["dr"],
+                               cfg["kmax"], cfg["dk"], cfg["maxell"])
+    ref.fast_hist = True
+    t0 = time.perf_counter()
+    out = ref.autocorrelate_interlaced(da, db)
+    dt = time.perf_counter() - t0
+    return dt, threads, out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the correlation path here is single-GPU "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from fftcorr_amd import _lib as lib
+    from fftcorr_amd import correlate as corr_mod
+    from fftcorr_amd.catalog import thomas_catalog
+
+    cfg = CFG
+    ctx = lib.context(0)
+    log("generating %d-parent Thomas catalogue" % cfg["parents"])
+    pos = thomas_catalog(cfg["parents"], cfg["children"], cfg["sigma"], 0.0, cfg["box"],
+                         cfg["seed"])
+    pos = np.ascontiguousarray(pos, np.float64)
+    log("particles: %d" % pos.shape[0])
+
+    log("FP32-mixed form: %d warmup + %d timed steps" % (args.warmup, args.steps))
+    f32, g32, c32 = run_form(lib, corr_mod, ctx, pos, cfg, "fp32", args.steps, args.warmup,
+                             profile=True)
+    del g32
+    fp64_steps = args.fp64_steps or max(3, args.steps // 4)
+    log("FP64 plain form: 1 warmup + %d timed steps" % fp64_steps)
+    f64, g64, c64 = run_form(lib, corr_mod, ctx, pos, cfg, "fp64", fp64_steps, 1, profile=False)
+
+    value = args.steps / f32["wall_s"]
+    value64 = fp64_steps / f64["wall_s"]
+    err = {}
+    for key, tag in (("k_hist", "P"), ("r_hist", "xi")):
+        e = floor_err(f32["res"][key], f64["res"][key])
+        for q in range(e.shape[0]):
+            err["%s_%d" % (tag, 2 * q)] = float(e[q].max())
+    counts_equal = bool(np.array_equal(f32["res"]["k_counts"], f64["res"]["k_counts"])
+                        and np.array_equal(f32["res"]["r_counts"], f64["res"]["r_counts"]))
+
+    # Roofline of the dominant stage (HIP events on the correlator's stream).
+    ks = int(np.prod(f32["kshape"]))
+    rs = int(np.prod(f32["rshape"]))
+    sb = stage_bytes(cfg, 4, 4, ks, rs)
+    st = {k: v for k, v in f32["stages"].items() if v[1] > 0}
+    dom = max(st, key=lambda k: st[k][0])
+    avg_ms = st[dom][0] / st[dom][1]
+    achieved = sb[dom] / (avg_ms * 1e-3) / 1e9 if dom in sb else None
+    stage_report = {k: {"ms_per_launch": v[0] / v[1], "launches_per_step": v[1] / args.steps,
+                        "GBps": (sb[k] / (v[0] / v[1] * 1e-3) / 1e9) if k in sb else None}
+                    for k, v in st.items()}
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline: FP64 oracle on one correlation")
+        dt, threads, ref = cpu_baseline(g64, cfg)
+        e_cpu = max(float(floor_err(f64["res"]["k_hist"], ref["k_hist"]).max()),
+                    float(floor_err(f64["res"]["r_hist"], ref["r_hist"]).max()))
+        cpu = {"value": 1.0 / dt, "unit": "correlations/s", "cores": threads, "kind": "port",
+               "sample": "one interlaced 512^3 l<=4 correlation of the same grids, FP64 "
+                         "numpy/scipy oracle (scipy.fft workers=%d), %.1f s" % (threads, dt),
+               "gpu_fp64_vs_cpu_max_floor_rel_err": e_cpu}
+
+    lat = f32["lat_ms"]
+    line = {
+        "metric": "correlations/s @512^3 l<=4, FP32-mixed vs FP64 plain at <=1e-5 rel err; p50/p99",
+        "value": value,
+        "unit": "correlations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * f32["wall_s"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "periodic P_l(k) + xi_l(r), l<=4, 512^3, CIC + interlacing, "
+                        "8M Thomas points in a 2000 Mpc/h box, kmax 0.5 dk 0.005, "
+                        "rmax 200 dr 5",
+            "grid": cfg["n"], "maxell": cfg["maxell"], "particles": int(pos.shape[0]),
+            "k_subgrid": list(f32["kshape"]), "r_subgrid": list(f32["rshape"]),
+            "precision": "FP32 grids/FFTs/products, FP64 Y_lm/windows/bins/sums",
+        },
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "fp64_plain": {"value": value64, "ms_per_step": 1e3 * f64["wall_s"] / fp64_steps,
+                       "p50_ms": float(np.percentile(f64["lat_ms"], 50))},
+        "fp32_over_fp64": value / value64,
+        "fp32_vs_fp64_max_floor_rel_err": err,
+        "counts_bit_exact": counts_equal,
+        "mass_assign_ms_two_grids": f32["ma_ms"],
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": (achieved / PEAK_HBM_GBS) if achieved else None,
+                     "traffic": None},
+        "stages": stage_report,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

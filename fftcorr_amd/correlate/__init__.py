@@ -1,0 +1,2 @@
+from fftcorr_amd.correlate.correlator import Correlator, PeriodicCorrelator
+from fftcorr_amd.correlate.utils import compute_xi

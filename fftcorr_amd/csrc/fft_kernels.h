@@ -1,0 +1,452 @@
+// LDS-staged Stockham FFT kernels for the 3D R2C / C2R transforms of the
+// padded work grid [n0][n1][n2/2+1] (the layout of cc/grid/fft_grid.cc:10-50).
+//
+//  * z pass (rows, contiguous): a real row of n2 samples is packed as n2/2
+//    complex values, transformed with a half-length complex FFT and
+//    post-processed into the n2/2+1 Hermitian half (and the reverse for C2R).
+//    Rows are loaded coalesced; the loader functor fuses the grid copy
+//    (array_ops::copy_into_padded_array) and, for the Yamamoto correlator,
+//    the Y_lm(x) weighting of the density into this load.
+//  * y and x passes (columns, strided): a workgroup owns TC adjacent
+//    columns (contiguous along z), so each global access is a run of TC
+//    complex values; the TC transforms run in LDS with E complex values per
+//    thread in registers between stages.
+//
+// Lengths 2^a (8..4096) and 3*2^a (24..3072) use compile-time radix plans;
+// every other length falls back to a direct O(L^2) DFT per column/row (used
+// for small or unusual test grids).  Forward transforms use sign -1 and no
+// normalisation, the inverse sign +1 and no 1/N, matching FFTW's r2c/c2r as
+// called by the reference (cc/grid/fft_grid.cc:81-84).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fc {
+
+template <typename T>
+struct CplxOf;
+template <>
+struct CplxOf<float> {
+  using type = float2;
+};
+template <>
+struct CplxOf<double> {
+  using type = double2;
+};
+template <typename T>
+using cplx = typename CplxOf<T>::type;
+
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) {
+  return C{a.x + b.x, a.y + b.y};
+}
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) {
+  return C{a.x - b.x, a.y - b.y};
+}
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) {
+  return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * (DIR * i): DIR = -1 multiplies by -i, DIR = +1 by +i.
+template <int DIR, typename C>
+__device__ __forceinline__ C mul_i(C a) {
+  return DIR < 0 ? C{a.y, -a.x} : C{-a.y, a.x};
+}
+
+// ---- register butterflies (radix R, sign DIR) -----------------------------
+template <int R, int DIR, typename T>
+struct Bfly;
+
+template <int DIR, typename T>
+struct Bfly<2, DIR, T> {
+  using C = cplx<T>;
+  static __device__ __forceinline__ void run(C* v) {
+    C a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <int DIR, typename T>
+struct Bfly<4, DIR, T> {
+  using C = cplx<T>;
+  static __device__ __forceinline__ void run(C* v) {
+    C t0 = cadd(v[0], v[2]);
+    C t1 = csub(v[0], v[2]);
+    C t2 = cadd(v[1], v[3]);
+    C t3 = mul_i<DIR>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+template <int DIR, typename T>
+struct Bfly<8, DIR, T> {
+  using C = cplx<T>;
+  static __device__ __forceinline__ void run(C* v) {
+    C e[4] = {v[0], v[2], v[4], v[6]};
+    C o[4] = {v[1], v[3], v[5], v[7]};
+    Bfly<4, DIR, T>::run(e);
+    Bfly<4, DIR, T>::run(o);
+    const T h = T(0.70710678118654752440084436210484903928);
+    // W8^1 = h (1 + DIR i), W8^2 = DIR i, W8^3 = h (-1 + DIR i)
+    C o1 = C{h * (o[1].x - DIR * o[1].y), h * (o[1].y + DIR * o[1].x)};
+    C o2 = mul_i<DIR>(o[2]);
+    C o3 = C{h * (-o[3].x - DIR * o[3].y), h * (-o[3].y + DIR * o[3].x)};
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+  }
+};
+
+template <int DIR, typename T>
+struct Bfly<3, DIR, T> {
+  using C = cplx<T>;
+  static __device__ __forceinline__ void run(C* v) {
+    const T s3 = T(0.86602540378443864676372317075293618347);  // sqrt(3)/2
+    C t = cadd(v[1], v[2]);
+    C d = mul_i<DIR>(csub(v[1], v[2]));
+    C m = C{v[0].x - T(0.5) * t.x, v[0].y - T(0.5) * t.y};
+    v[0] = cadd(v[0], t);
+    v[1] = C{m.x + s3 * d.x, m.y + s3 * d.y};
+    v[2] = C{m.x - s3 * d.x, m.y - s3 * d.y};
+  }
+};
+
+template <int... Rs>
+struct Radices {};
+
+// Compile-time plans: E = complex values per thread (divisible by every
+// radix), so a length-L column uses L/E threads.
+template <int L>
+struct FftCfg {
+  static constexpr bool fast = false;
+};
+#define FC_FFT_CFG(Lv, Ev, ...)                 \
+  template <>                                   \
+  struct FftCfg<Lv> {                           \
+    static constexpr bool fast = true;          \
+    static constexpr int E = Ev;                \
+    using radices = Radices<__VA_ARGS__>;       \
+  };
+FC_FFT_CFG(8, 8, 8)
+FC_FFT_CFG(16, 8, 8, 2)
+FC_FFT_CFG(32, 8, 8, 4)
+FC_FFT_CFG(64, 8, 8, 8)
+FC_FFT_CFG(128, 8, 8, 8, 2)
+FC_FFT_CFG(256, 8, 8, 8, 4)
+FC_FFT_CFG(512, 8, 8, 8, 8)
+FC_FFT_CFG(1024, 8, 8, 8, 8, 2)
+FC_FFT_CFG(2048, 8, 8, 8, 8, 4)
+FC_FFT_CFG(4096, 8, 8, 8, 8, 8)
+FC_FFT_CFG(24, 24, 8, 3)
+FC_FFT_CFG(48, 24, 8, 2, 3)
+FC_FFT_CFG(96, 24, 8, 4, 3)
+FC_FFT_CFG(192, 24, 8, 8, 3)
+FC_FFT_CFG(384, 24, 8, 8, 2, 3)
+FC_FFT_CFG(768, 24, 8, 8, 4, 3)
+FC_FFT_CFG(1536, 24, 8, 8, 8, 3)
+FC_FFT_CFG(3072, 24, 8, 8, 8, 2, 3)
+#undef FC_FFT_CFG
+
+// LDS addressing: element e of column c.
+struct LayColFast {  // [e][TC]: columns contiguous (strided y/x passes)
+  int tc;
+  __device__ __forceinline__ int operator()(int c, int e) const { return e * tc + c; }
+};
+struct LayRow {  // [c][rs]: elements contiguous (row / z pass)
+  int rs;
+  __device__ __forceinline__ int operator()(int c, int e) const { return c * rs + e; }
+};
+
+// One Stockham stage of radix R on LDS column c, thread t of L/E.
+// Twiddle table tw[m] = exp(-2 pi i m / L), m < L.
+template <typename T, int L, int E, int R, int NS, int DIR, class Lay>
+__device__ __forceinline__ void stockham_stage(cplx<T>* lds, const cplx<T>* __restrict__ tw,
+                                               int c, int t, Lay lay) {
+  using C = cplx<T>;
+  constexpr int TCOL = L / E;
+  constexpr int NB = E / R;
+  constexpr int LR = L / R;
+  C v[E];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * TCOL;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[b * R + r] = lds[lay(c, j + r * LR)];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * TCOL;
+    const int k = j % NS;
+    if (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        C w = tw[r * k * (L / (NS * R))];
+        if (DIR > 0) w.y = -w.y;
+        v[b * R + r] = cmul(v[b * R + r], w);
+      }
+    }
+    Bfly<R, DIR, T>::run(&v[b * R]);
+    const int base = (j / NS) * NS * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds[lay(c, base + r * NS)] = v[b * R + r];
+  }
+  __syncthreads();
+}
+
+template <typename T, int L, int E, int DIR, int NS, class Lay>
+__device__ __forceinline__ void run_stages(Radices<>, cplx<T>*, const cplx<T>*, int, int, Lay) {}
+
+template <typename T, int L, int E, int DIR, int NS, class Lay, int R, int... Rest>
+__device__ __forceinline__ void run_stages(Radices<R, Rest...>, cplx<T>* lds,
+                                           const cplx<T>* __restrict__ tw, int c, int t,
+                                           Lay lay) {
+  stockham_stage<T, L, E, R, NS, DIR>(lds, tw, c, t, lay);
+  run_stages<T, L, E, DIR, NS * R>(Radices<Rest...>{}, lds, tw, c, t, lay);
+}
+
+template <typename T, int L, int DIR, class Lay>
+__device__ __forceinline__ void fft_lds(cplx<T>* lds, const cplx<T>* __restrict__ tw, int c,
+                                        int t, Lay lay) {
+  run_stages<T, L, FftCfg<L>::E, DIR, 1>(typename FftCfg<L>::radices{}, lds, tw, c, t, lay);
+}
+
+// ---- strided column pass ---------------------------------------------------
+// data is [A][L][B] complex with the transform along L:
+//   element (a, e, b) at a*strideA + e*strideL + b.
+// grid = (ceil(B / tc), A); block = tc * (L / E) threads; LDS = L * tc complex.
+template <typename T, int L, int DIR>
+__global__ void __launch_bounds__(1024) k_fft_cols(cplx<T>* __restrict__ data, int64_t strideA,
+                                                   int64_t strideL, int B, int tc,
+                                                   const cplx<T>* __restrict__ tw) {
+  using C = cplx<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  const int b0 = blockIdx.x * tc;
+  C* base = data + (int64_t)blockIdx.y * strideA + b0;
+  const int nthr = blockDim.x;
+  const int tid = threadIdx.x;
+  const int ncol = min(tc, B - b0);
+  for (int idx = tid; idx < L * tc; idx += nthr) {
+    const int e = idx / tc, c = idx - e * tc;
+    lds[idx] = c < ncol ? base[e * strideL + c] : C{T(0), T(0)};
+  }
+  __syncthreads();
+  fft_lds<T, L, DIR>(lds, tw, tid % tc, tid / tc, LayColFast{tc});
+  for (int idx = tid; idx < L * tc; idx += nthr) {
+    const int e = idx / tc, c = idx - e * tc;
+    if (c < ncol) base[e * strideL + c] = lds[idx];
+  }
+}
+
+// Direct DFT fallback for one column per block (any L up to LDS capacity).
+template <typename T, int DIR>
+__global__ void __launch_bounds__(256) k_dft_cols(cplx<T>* __restrict__ data, int64_t strideA,
+                                                  int64_t strideL, int B, int L,
+                                                  const cplx<T>* __restrict__ tw) {
+  using C = cplx<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  C* base = data + (int64_t)blockIdx.y * strideA + blockIdx.x;
+  for (int e = threadIdx.x; e < L; e += blockDim.x) lds[e] = base[e * strideL];
+  __syncthreads();
+  for (int k = threadIdx.x; k < L; k += blockDim.x) {
+    T sx = 0, sy = 0;
+    int idx = 0;
+    for (int e = 0; e < L; ++e) {
+      C w = tw[idx];
+      if (DIR > 0) w.y = -w.y;
+      const C x = lds[e];
+      sx += x.x * w.x - x.y * w.y;
+      sy += x.x * w.y + x.y * w.x;
+      idx += k;
+      if (idx >= L) idx -= L;
+    }
+    base[k * strideL] = C{sx, sy};  // each block owns its column
+  }
+}
+
+// ---- row (z) pass: real <-> Hermitian half --------------------------------
+// Loader: T operator()(double v, int64 row, int k) turns the input sample
+// (k-th of the row) into the value transformed.  Rows are [nrows][ldi] (input
+// type Tin) and [nrows][ldo] complex outputs.
+struct LoadCopy {
+  __device__ __forceinline__ double operator()(double v, int64_t, int) const { return v; }
+};
+
+// block = tr * (M / E) threads, handling rows blockIdx.x*tr ... ; LDS tr*(M+1).
+template <typename T, typename Tin, int M, class Loader>
+__global__ void __launch_bounds__(1024) k_fft_rows_r2c(const Tin* __restrict__ in, int64_t ldi,
+                                                       cplx<T>* __restrict__ out, int64_t ldo,
+                                                       int64_t nrows, int tr,
+                                                       const cplx<T>* __restrict__ twM,
+                                                       const cplx<T>* __restrict__ twN,
+                                                       Loader ld) {
+  using C = cplx<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  constexpr int RS = M + 1;
+  const int64_t row0 = (int64_t)blockIdx.x * tr;
+  const int nthr = blockDim.x, tid = threadIdx.x;
+  for (int idx = tid; idx < tr * M; idx += nthr) {
+    const int c = idx / M, m = idx - c * M;
+    const int64_t row = row0 + c;
+    C z{T(0), T(0)};
+    if (row < nrows) {
+      const Tin* p = in + row * ldi + 2 * m;
+      z.x = T(ld((double)p[0], row, 2 * m));
+      z.y = T(ld((double)p[1], row, 2 * m + 1));
+    }
+    lds[c * RS + m] = z;
+  }
+  __syncthreads();
+  constexpr int TCOL = M / FftCfg<M>::E;
+  fft_lds<T, M, -1>(lds, twM, tid / TCOL, tid % TCOL, LayRow{RS});
+  // Hermitian half: X[k] = (Z[k] + conj Z[M-k])/2 - i W^k (Z[k] - conj Z[M-k])/2
+  for (int idx = tid; idx < tr * (M + 1); idx += nthr) {
+    const int c = idx / (M + 1), k = idx - c * (M + 1);
+    const int64_t row = row0 + c;
+    if (row >= nrows) continue;
+    const C zk = lds[c * RS + (k == M ? 0 : k)];
+    const C zm = lds[c * RS + (k == 0 ? 0 : M - k)];
+    const T ax = zk.x + zm.x, ay = zk.y - zm.y;
+    const T bx = zk.x - zm.x, by = zk.y + zm.y;
+    const C w = twN[k];
+    const T px = w.x * by + w.y * bx;
+    const T py = -(w.x * bx - w.y * by);
+    out[row * ldo + k] = C{T(0.5) * (ax + px), T(0.5) * (ay + py)};
+  }
+}
+
+// Inverse: Hermitian half rows -> real rows (unnormalised; imaginary parts of
+// the DC and Nyquist terms are ignored, as FFTW's c2r does).
+template <typename T, typename Tout, int M>
+__global__ void __launch_bounds__(1024) k_fft_rows_c2r(const cplx<T>* __restrict__ in, int64_t ldi,
+                                                       Tout* __restrict__ out, int64_t ldo,
+                                                       int64_t nrows, int tr,
+                                                       const cplx<T>* __restrict__ twM,
+                                                       const cplx<T>* __restrict__ twN) {
+  using C = cplx<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  constexpr int RS = M + 1;
+  const int64_t row0 = (int64_t)blockIdx.x * tr;
+  const int nthr = blockDim.x, tid = threadIdx.x;
+  for (int idx = tid; idx < tr * (M + 1); idx += nthr) {
+    const int c = idx / (M + 1), k = idx - c * (M + 1);
+    const int64_t row = row0 + c;
+    C x{T(0), T(0)};
+    if (row < nrows) x = in[row * ldi + k];
+    if (k == 0 || k == M) x.y = T(0);
+    lds[c * RS + k] = x;
+  }
+  __syncthreads();
+  // Z[k] = (X[k] + conj X[M-k]) + i W^-k (X[k] - conj X[M-k]) for k < M,
+  // computed in pairs (p, M-p) so the update can be in place.
+  for (int idx = tid; idx < tr * (M / 2 + 1); idx += nthr) {
+    const int c = idx / (M / 2 + 1), p = idx - c * (M / 2 + 1);
+    const C xp = lds[c * RS + p];
+    const C xq = lds[c * RS + (M - p)];
+    const C wp = twN[p];       // exp(-2 pi i p / N); use conj -> W^-p
+    const C wq = twN[M - p];
+    // Z = A + i conj(w) B with A = X[k] + conj X[M-k], B = X[k] - conj X[M-k];
+    // conj(w) B = (w.x bx + w.y by, w.x by - w.y bx), and i (cx + i cy) = (-cy, cx).
+    {
+      const T ax = xp.x + xq.x, ay = xp.y - xq.y;
+      const T bx = xp.x - xq.x, by = xp.y + xq.y;
+      const T cx = wp.x * bx + wp.y * by, cy = wp.x * by - wp.y * bx;
+      lds[c * RS + p] = C{ax - cy, ay + cx};
+    }
+    if (p != 0 && p != M - p) {
+      const T ax = xq.x + xp.x, ay = xq.y - xp.y;
+      const T bx = xq.x - xp.x, by = xq.y + xp.y;
+      const T cx = wq.x * bx + wq.y * by, cy = wq.x * by - wq.y * bx;
+      lds[c * RS + (M - p)] = C{ax - cy, ay + cx};
+    }
+  }
+  __syncthreads();
+  constexpr int TCOL = M / FftCfg<M>::E;
+  fft_lds<T, M, +1>(lds, twM, tid / TCOL, tid % TCOL, LayRow{RS});
+  for (int idx = tid; idx < tr * M; idx += nthr) {
+    const int c = idx / M, m = idx - c * M;
+    const int64_t row = row0 + c;
+    if (row >= nrows) continue;
+    const C z = lds[c * RS + m];
+    Tout* p = out + row * ldo + 2 * m;
+    p[0] = Tout(z.x);
+    p[1] = Tout(z.y);
+  }
+}
+
+// Direct-DFT fallback rows (any n2): one block per row.
+template <typename T, typename Tin, class Loader>
+__global__ void __launch_bounds__(256) k_dft_rows_r2c(const Tin* __restrict__ in, int64_t ldi,
+                                                      cplx<T>* __restrict__ out, int64_t ldo,
+                                                      int n2, const cplx<T>* __restrict__ tw,
+                                                      Loader ld) {
+  using C = cplx<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* x = reinterpret_cast<T*>(smem_raw);
+  const int64_t row = blockIdx.x;
+  for (int n = threadIdx.x; n < n2; n += blockDim.x)
+    x[n] = T(ld((double)in[row * ldi + n], row, n));
+  __syncthreads();
+  // Every global read of the row happened before the barrier above, so the
+  // output row may alias the input row (in-place transform).
+  const int nh = n2 / 2 + 1;
+  for (int k = threadIdx.x; k < nh; k += blockDim.x) {
+    T sx = 0, sy = 0;
+    int idx = 0;
+    for (int n = 0; n < n2; ++n) {
+      const C w = tw[idx];
+      sx += x[n] * w.x;
+      sy += x[n] * w.y;
+      idx += k;
+      if (idx >= n2) idx -= n2;
+    }
+    out[row * ldo + k] = C{sx, sy};
+  }
+}
+
+template <typename T, typename Tout>
+__global__ void __launch_bounds__(256) k_dft_rows_c2r(const cplx<T>* __restrict__ in, int64_t ldi,
+                                                      Tout* __restrict__ out, int64_t ldo, int n2,
+                                                      const cplx<T>* __restrict__ tw) {
+  using C = cplx<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* X = reinterpret_cast<C*>(smem_raw);
+  const int64_t row = blockIdx.x;
+  const int nh = n2 / 2 + 1;
+  for (int k = threadIdx.x; k < nh; k += blockDim.x) {
+    C v = in[row * ldi + k];
+    if (k == 0 || (2 * k == n2)) v.y = T(0);
+    X[k] = v;
+  }
+  __syncthreads();
+  for (int n = threadIdx.x; n < n2; n += blockDim.x) {
+    T s = 0;
+    int idx = 0;
+    for (int k = 0; k < n2; ++k) {
+      const C xk = k < nh ? X[k] : C{X[n2 - k].x, -X[n2 - k].y};
+      const C w = tw[idx];  // exp(-2 pi i n k / n2); the inverse uses conj(w)
+      s += xk.x * w.x + xk.y * w.y;
+      idx += n;
+      if (idx >= n2) idx -= n2;
+    }
+    out[row * ldo + n] = Tout(s);
+  }
+}
+
+}  // namespace fc

@@ -1,0 +1,201 @@
+// Host side of the 3D R2C/C2R FFT: plan (twiddle tables + tile shapes per
+// axis) and launchers.  A plan replaces FftGrid::plan_fft
+// (cc/grid/fft_grid.cc:69-115); the execute calls replace execute_fft /
+// execute_ifft (:125-178), split into the z (rows) and y/x (columns) passes so
+// callers can fuse their own loads into the z pass.
+#pragma once
+
+#include "fc_internal.h"
+#include "fft_kernels.h"
+
+namespace fc {
+
+constexpr int kMaxFftLen = 4096;
+constexpr size_t kLdsBudget = 65536;  // bytes of LDS per workgroup we allow
+
+// Tile shape chosen for one transform axis (autotuned; see fft.hip).
+struct AxisTiles {
+  int tc = 0;  // columns (or rows) per workgroup
+};
+
+struct FftPlan {
+  int device = 0;
+  int n[3] = {0, 0, 0};
+  int n2h = 0;
+  int prec = FC_FP64;
+  bool fast[3] = {false, false, false};  // axis 2: the half-length row FFT
+  int tile[3] = {0, 0, 0};
+  // Twiddle tables exp(-2 pi i m / L) in the plan's precision.
+  DeviceBuffer tw0, tw1;      // lengths n0, n1
+  DeviceBuffer tw2M, tw2N;    // lengths n2/2 (fast rows) and n2 (post-process / DFT)
+  size_t csize() const { return prec == FC_FP64 ? sizeof(double2) : sizeof(float2); }
+  int64_t nrows() const { return (int64_t)n[0] * n[1]; }
+  size_t complex_bytes() const { return (size_t)n[0] * n[1] * n2h * csize(); }
+};
+
+bool fft_len_fast(int L);
+int fft_len_e(int L);
+
+// Plan cache keyed by (device, shape, precision).
+std::shared_ptr<FftPlan> get_fft_plan(int device, const int shape[3], int prec);
+
+// ---- launchers -------------------------------------------------------------
+template <typename T, int DIR>
+void launch_cols(int L, cplx<T>* data, int64_t strideA, int A, int64_t strideL, int B, int tc,
+                 const cplx<T>* tw, hipStream_t s) {
+  const size_t cs = sizeof(cplx<T>);
+#define FC_COLS_CASE(Lv)                                                                 \
+  case Lv: {                                                                             \
+    const int thr = tc * (Lv / FftCfg<Lv>::E);                                           \
+    dim3 g((B + tc - 1) / tc, A);                                                        \
+    hipLaunchKernelGGL((k_fft_cols<T, Lv, DIR>), g, dim3(thr), (size_t)Lv * tc * cs, s,  \
+                       data, strideA, strideL, B, tc, tw);                               \
+    break;                                                                               \
+  }
+  switch (L) {
+    FC_COLS_CASE(8)
+    FC_COLS_CASE(16)
+    FC_COLS_CASE(32)
+    FC_COLS_CASE(64)
+    FC_COLS_CASE(128)
+    FC_COLS_CASE(256)
+    FC_COLS_CASE(512)
+    FC_COLS_CASE(1024)
+    FC_COLS_CASE(2048)
+    FC_COLS_CASE(4096)
+    FC_COLS_CASE(24)
+    FC_COLS_CASE(48)
+    FC_COLS_CASE(96)
+    FC_COLS_CASE(192)
+    FC_COLS_CASE(384)
+    FC_COLS_CASE(768)
+    FC_COLS_CASE(1536)
+    FC_COLS_CASE(3072)
+    default: {
+      dim3 g(B, A);
+      hipLaunchKernelGGL((k_dft_cols<T, DIR>), g, dim3(256), (size_t)L * cs, s, data, strideA,
+                         strideL, B, L, tw);
+    }
+  }
+#undef FC_COLS_CASE
+  FC_LAUNCH_CHECK();
+}
+
+template <typename T, typename Tin, class Loader>
+void launch_rows_r2c(const FftPlan& p, const Tin* in, int64_t ldi, cplx<T>* out, int64_t ldo,
+                     Loader ld, hipStream_t s) {
+  const size_t cs = sizeof(cplx<T>);
+  const int64_t nrows = p.nrows();
+  const int M = p.n[2] / 2;
+  const int tr = p.tile[2];
+  const cplx<T>* twM = p.tw2M.as<cplx<T>>();
+  const cplx<T>* twN = p.tw2N.as<cplx<T>>();
+  if (p.fast[2]) {
+#define FC_R2C_CASE(Mv)                                                                     \
+  case Mv: {                                                                                \
+    const int thr = tr * (Mv / FftCfg<Mv>::E);                                              \
+    dim3 g((unsigned)((nrows + tr - 1) / tr));                                              \
+    hipLaunchKernelGGL((k_fft_rows_r2c<T, Tin, Mv, Loader>), g, dim3(thr),                  \
+                       (size_t)tr * (Mv + 1) * cs, s, in, ldi, out, ldo, nrows, tr, twM,     \
+                       twN, ld);                                                            \
+    break;                                                                                  \
+  }
+    switch (M) {
+      FC_R2C_CASE(8)
+      FC_R2C_CASE(16)
+      FC_R2C_CASE(32)
+      FC_R2C_CASE(64)
+      FC_R2C_CASE(128)
+      FC_R2C_CASE(256)
+      FC_R2C_CASE(512)
+      FC_R2C_CASE(1024)
+      FC_R2C_CASE(2048)
+      FC_R2C_CASE(24)
+      FC_R2C_CASE(48)
+      FC_R2C_CASE(96)
+      FC_R2C_CASE(192)
+      FC_R2C_CASE(384)
+      FC_R2C_CASE(768)
+      FC_R2C_CASE(1536)
+      default:
+        fail(FC_ERR_RUNTIME, "internal: no fast row kernel for this length");
+    }
+#undef FC_R2C_CASE
+  } else {
+    dim3 g((unsigned)nrows);
+    hipLaunchKernelGGL((k_dft_rows_r2c<T, Tin, Loader>), g, dim3(256),
+                       (size_t)p.n[2] * sizeof(T), s, in, ldi, out, ldo, p.n[2], twN, ld);
+  }
+  FC_LAUNCH_CHECK();
+}
+
+template <typename T, typename Tout>
+void launch_rows_c2r(const FftPlan& p, const cplx<T>* in, int64_t ldi, Tout* out, int64_t ldo,
+                     hipStream_t s) {
+  const size_t cs = sizeof(cplx<T>);
+  const int64_t nrows = p.nrows();
+  const int M = p.n[2] / 2;
+  const int tr = p.tile[2];
+  const cplx<T>* twM = p.tw2M.as<cplx<T>>();
+  const cplx<T>* twN = p.tw2N.as<cplx<T>>();
+  if (p.fast[2]) {
+#define FC_C2R_CASE(Mv)                                                                    \
+  case Mv: {                                                                               \
+    const int thr = tr * (Mv / FftCfg<Mv>::E);                                             \
+    dim3 g((unsigned)((nrows + tr - 1) / tr));                                             \
+    hipLaunchKernelGGL((k_fft_rows_c2r<T, Tout, Mv>), g, dim3(thr),                        \
+                       (size_t)tr * (Mv + 1) * cs, s, in, ldi, out, ldo, nrows, tr, twM,    \
+                       twN);                                                               \
+    break;                                                                                 \
+  }
+    switch (M) {
+      FC_C2R_CASE(8)
+      FC_C2R_CASE(16)
+      FC_C2R_CASE(32)
+      FC_C2R_CASE(64)
+      FC_C2R_CASE(128)
+      FC_C2R_CASE(256)
+      FC_C2R_CASE(512)
+      FC_C2R_CASE(1024)
+      FC_C2R_CASE(2048)
+      FC_C2R_CASE(24)
+      FC_C2R_CASE(48)
+      FC_C2R_CASE(96)
+      FC_C2R_CASE(192)
+      FC_C2R_CASE(384)
+      FC_C2R_CASE(768)
+      FC_C2R_CASE(1536)
+      default:
+        fail(FC_ERR_RUNTIME, "internal: no fast row kernel for this length");
+    }
+#undef FC_C2R_CASE
+  } else {
+    dim3 g((unsigned)nrows);
+    const int nh = p.n[2] / 2 + 1;
+    hipLaunchKernelGGL((k_dft_rows_c2r<T, Tout>), g, dim3(256), (size_t)nh * cs, s, in, ldi,
+                       out, ldo, p.n[2], twN);
+  }
+  FC_LAUNCH_CHECK();
+}
+
+// y then x forward passes over the complex work grid [n0][n1][n2h].
+template <typename T>
+void fft_yx_forward(const FftPlan& p, cplx<T>* data, hipStream_t s) {
+  const int64_t n2h = p.n2h;
+  launch_cols<T, -1>(p.n[1], data, (int64_t)p.n[1] * n2h, p.n[0], n2h, (int)n2h, p.tile[1],
+                     p.tw1.as<cplx<T>>(), s);
+  launch_cols<T, -1>(p.n[0], data, 0, 1, (int64_t)p.n[1] * n2h, (int)(p.n[1] * n2h),
+                     p.tile[0], p.tw0.as<cplx<T>>(), s);
+}
+
+// x then y inverse passes.
+template <typename T>
+void fft_xy_inverse(const FftPlan& p, cplx<T>* data, hipStream_t s) {
+  const int64_t n2h = p.n2h;
+  launch_cols<T, +1>(p.n[0], data, 0, 1, (int64_t)p.n[1] * n2h, (int)(p.n[1] * n2h),
+                     p.tile[0], p.tw0.as<cplx<T>>(), s);
+  launch_cols<T, +1>(p.n[1], data, (int64_t)p.n[1] * n2h, p.n[0], n2h, (int)n2h, p.tile[1],
+                     p.tw1.as<cplx<T>>(), s);
+}
+
+}  // namespace fc

@@ -1,0 +1,144 @@
+// Deterministic wavefront-reduced histogramming (HistogramList::accumulate,
+// cc/histogram/histogram_list.h:33-45) for NV values per sample.
+//
+// Within a 64-lane wave, the lanes that share a bin are summed with a fixed
+// xor butterfly and the leader lane adds the sum to that wave's private LDS
+// bins; waves are then combined in wave order into one partial per block, and
+// blocks (fixed count, contiguous sample ranges) in block order by
+// k_hist_final.  Counts are integers (exact in any order).  The summation
+// order therefore depends only on the problem shape, never on scheduling.
+#pragma once
+
+#include "fc_objects.h"
+
+namespace fc {
+
+constexpr int kHistBlocks = 1024;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Adds (bin, v[NV]) of every lane of the calling wave; bin < 0 skips.
+// wbins: this wave's [NV][nbins] LDS bins; lcounts: the block's [nbins] counts.
+template <int NV>
+__device__ __forceinline__ void wave_hist_add(int bin, const double* v, double* wbins,
+                                              int* lcounts, int nbins) {
+  const int lane = threadIdx.x % kWave;
+  unsigned long long pending = __ballot(bin >= 0);
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const int lb = __shfl(bin, leader, kWave);
+    const bool mine = (bin == lb);
+    const unsigned long long m = __ballot(mine);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const double s = wave_sum(mine ? v[q] : 0.0);
+      if (lane == leader) wbins[q * nbins + lb] += s;
+    }
+    if (lane == leader) atomicAdd(&lcounts[lb], (int)__popcll(m));  // shared by waves
+    pending &= ~m;
+  }
+}
+
+// LDS bytes for a block of `threads` threads.
+inline size_t hist_lds_bytes(int nv, int nbins, int threads) {
+  return sizeof(double) * (size_t)(threads / kWave) * nv * nbins + sizeof(int) * (size_t)nbins;
+}
+
+// Picks 256 threads if the per-wave bins fit, else 64; 0 if even one wave's
+// bins do not fit in 64 KB.
+inline int hist_threads(int nv, int nbins) {
+  if (hist_lds_bytes(nv, nbins, 256) <= 65536) return 256;
+  if (hist_lds_bytes(nv, nbins, 64) <= 65536) return 64;
+  return 0;
+}
+
+// Generic histogram kernel: F(idx, &bin, v) fills bin (or -1) and NV values
+// for sample idx < n.  part_hist [gridDim][NV][nbins], part_counts [gridDim][nbins].
+template <int NV, class F>
+__global__ void __launch_bounds__(256)
+    k_hist(F f, int64_t n, int nbins, double* __restrict__ part_hist,
+           int* __restrict__ part_counts) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int nw = blockDim.x / kWave;
+  double* bins = reinterpret_cast<double*>(smem_raw);
+  int* lcounts = reinterpret_cast<int*>(bins + (size_t)nw * NV * nbins);
+  for (int i = threadIdx.x; i < nw * NV * nbins; i += blockDim.x) bins[i] = 0.0;
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) lcounts[i] = 0;
+  __syncthreads();
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < n ? lo + chunk : n;
+  double* wb = bins + (size_t)(threadIdx.x / kWave) * NV * nbins;
+  const int64_t rounds = hi > lo ? (hi - lo + blockDim.x - 1) / blockDim.x : 0;
+  for (int64_t r = 0; r < rounds; ++r) {
+    const int64_t idx = lo + r * blockDim.x + threadIdx.x;
+    int bin = -1;
+    double v[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = 0.0;
+    if (idx < hi) f(idx, &bin, v);
+    wave_hist_add<NV>(bin, v, wb, lcounts, nbins);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NV * nbins; i += blockDim.x) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += bins[(size_t)w * NV * nbins + i];
+    part_hist[(size_t)blockIdx.x * NV * nbins + i] = s;
+  }
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x)
+    part_counts[(size_t)blockIdx.x * nbins + i] = lcounts[i];
+}
+
+// hist[row0 + q][b] += sum over blocks (in order) of part_hist[blk][q][b];
+// counts[row][b] += sum of part_counts for every row in [crow0, crow1).
+template <int NV>
+__global__ void k_hist_final(const double* __restrict__ part_hist,
+                             const int* __restrict__ part_counts, int nblocks, int nbins,
+                             double* __restrict__ hist, int* __restrict__ counts, int row0,
+                             int crow0, int crow1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < NV * nbins) {
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part_hist[(size_t)b * NV * nbins + i];
+    const int q = i / nbins, bin = i % nbins;
+    hist[(size_t)(row0 + q) * nbins + bin] += s;
+  }
+  if (i < nbins) {
+    int c = 0;
+    for (int b = 0; b < nblocks; ++b) c += part_counts[(size_t)b * nbins + i];
+    for (int r = crow0; r < crow1; ++r) counts[(size_t)r * nbins + i] += c;
+  }
+}
+
+// Scratch for one histogram pass.
+struct HistScratch {
+  DeviceBuffer part_hist, part_counts;
+  void reserve(int nv, int nbins) {
+    part_hist.reserve(sizeof(double) * (size_t)kHistBlocks * nv * nbins);
+    part_counts.reserve(sizeof(int) * (size_t)kHistBlocks * nbins);
+  }
+};
+
+template <int NV, class F>
+void run_hist(F f, int64_t n, int nbins, HistScratch& hs, double* hist, int* counts, int row0,
+              int crow0, int crow1, hipStream_t s) {
+  const int thr = hist_threads(NV, nbins);
+  if (thr == 0) fail(FC_ERR_INVALID, "too many histogram bins (" + std::to_string(nbins) + ")");
+  hs.reserve(NV, nbins);
+  int nb = (int)std::min<int64_t>(kHistBlocks, (n + thr - 1) / thr);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL((k_hist<NV, F>), dim3(nb), dim3(thr), hist_lds_bytes(NV, nbins, thr), s, f,
+                     n, nbins, hs.part_hist.as<double>(), hs.part_counts.as<int>());
+  FC_LAUNCH_CHECK();
+  const int tot = NV * nbins > nbins ? NV * nbins : nbins;
+  hipLaunchKernelGGL((k_hist_final<NV>), dim3((tot + 255) / 256), dim3(256), 0, s,
+                     hs.part_hist.as<double>(), hs.part_counts.as<int>(), nb, nbins, hist,
+                     counts, row0, crow0, crow1);
+  FC_LAUNCH_CHECK();
+}
+
+}  // namespace fc

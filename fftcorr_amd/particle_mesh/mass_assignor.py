@@ -1,0 +1,124 @@
+"""MassAssignor (fftcorr/particle_mesh/mass_assignor.pyx:18-109,
+cc/particle_mesh/mass_assignor.h:30-234) running on the GPU.
+
+Particles are buffered on the host (``buffer_size`` of them) and each flush
+is one device call: grid coordinates and skips, a stable sort by cell, and a
+deterministic gather of the NGP / TSC / CIC stencil (no float atomics).
+``num_added``, ``num_skipped``, ``totw`` and ``totwsq`` count flushed
+particles.  ``clear()`` also empties the buffer (the reference does so only
+in its OpenMP build).
+"""
+
+import numpy as np
+
+from fftcorr_amd import _lib
+from fftcorr_amd._arrays import check_row_major
+
+
+class MassAssignor:
+    def __init__(self, grid, distribution_scheme, periodic_wrap=False, buffer_size=0,
+                 shift=0.0):
+        from fftcorr_amd.grid import ConfigSpaceGrid
+        if not isinstance(grid, ConfigSpaceGrid):
+            raise TypeError("grid must be a ConfigSpaceGrid")
+        distribution_scheme = int(distribution_scheme)
+        if distribution_scheme not in (0, 1, 3):
+            raise ValueError("distribution scheme %d is not supported on the GPU "
+                             "(0 NGP, 1 TSC, 3 CIC)" % distribution_scheme)
+        if ("distribution_scheme" in grid.metadata
+                and grid.metadata["distribution_scheme"] != distribution_scheme):
+            grid.metadata["distribution_scheme"] = -1
+        else:
+            grid.metadata["distribution_scheme"] = distribution_scheme
+        self._grid = grid
+        self._scheme = distribution_scheme
+        self._wrap = bool(periodic_wrap)
+        self._buffer_size = int(buffer_size)
+        self._shift = float(shift)
+        self._pending = []
+        self._npending = 0
+        self._stats = _lib.MaStats()
+
+    @property
+    def buffer_size(self):
+        return self._buffer_size
+
+    def clear(self):
+        self._pending = []
+        self._npending = 0
+        self._stats = _lib.MaStats()
+
+    def add_particles(self, particles, weight=None):
+        self.add_particles_to_buffer(particles, weight)
+        self.flush()
+
+    def add_particles_to_buffer(self, particles, weight=None):
+        check_row_major(particles, 2)
+        if particles.shape[1] == 4:
+            if weight is not None:
+                raise ValueError("Cannot pass weights twice")
+            self._enqueue(particles, None, 0.0)
+            return
+        if particles.shape[1] != 3:
+            raise ValueError("Particles must have shape (n, 3) or (n, 4). "
+                             f"Got: {particles.shape}")
+        if weight is None:
+            weight = 1.0
+        weight = np.asarray(weight)
+        if not weight.shape:
+            self._enqueue(particles, None, float(weight))
+            return
+        if weight.shape != (particles.shape[0],):
+            raise ValueError("Weight must be 1D with the same length as particles. "
+                             f"Got {weight.shape} and {particles.shape}")
+        check_row_major(weight, 1)
+        self._enqueue(particles, weight, 0.0)
+
+    def _enqueue(self, pos, w, wscalar):
+        n = pos.shape[0]
+        if n == 0:
+            return
+        if self._buffer_size <= 0:
+            self._assign(pos, w, wscalar)
+            return
+        self._pending.append((pos.copy(), None if w is None else w.copy(), wscalar))
+        self._npending += n
+        if self._npending >= self._buffer_size:
+            self.flush()
+
+    def _assign(self, pos, w, wscalar):
+        dev = self._grid.device_grid()
+        check = _lib.check
+        check(_lib.lib().fc_mass_assign(
+            dev.handle, _lib.dptr(pos), pos.shape[0], pos.shape[1], _lib.dptr(w),
+            float(wscalar), self._scheme, int(self._wrap), self._shift, self._stats))
+        self._grid._device_modified()
+
+    def flush(self):
+        pending, self._pending, self._npending = self._pending, [], 0
+        for pos, w, wscalar in pending:
+            self._assign(pos, w, wscalar)
+
+    @property
+    def num_added(self):
+        return int(self._stats.num_added)
+
+    @property
+    def num_skipped(self):
+        return int(self._stats.num_skipped)
+
+    @property
+    def totw(self):
+        return float(self._stats.totw)
+
+    @property
+    def totwsq(self):
+        return float(self._stats.totwsq)
+
+    @property
+    def sort_time(self):
+        return float(self._stats.sort_time)
+
+    @property
+    def window_time(self):
+        return float(self._stats.window_time)

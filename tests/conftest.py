@@ -22,3 +22,28 @@ def ref():
     if _ref.load() is None:
         pytest.skip("oracle/_ref not built (needs the reference sources at build time)")
     return _ref
+
+
+def _gpu_present():
+    try:
+        from fftcorr_amd import _lib
+        return _lib.device_count() > 0
+    except RuntimeError as e:
+        if "not built" in str(e):
+            raise  # the library must exist wherever GPU tests run
+        return False
+
+
+def pytest_collection_modifyitems(config, items):
+    gpu_items = [it for it in items if "gpu" in it.keywords]
+    if not gpu_items or _gpu_present():
+        return
+    skip = pytest.mark.skip(reason="no GPU visible to HIP")
+    for it in gpu_items:
+        it.add_marker(skip)
+
+
+@pytest.fixture(scope="session")
+def fclib():
+    from fftcorr_amd import _lib
+    return _lib
