@@ -57,21 +57,27 @@ def make_grids(lib, ctx, pos, cfg, precision):
     return grids, ma_ms, st
 
 
-def stage_bytes(cfg, s_work, s_grid, ks, rs):
-    """Algorithmic HBM bytes per launch of each stage (complex half grid
-    n^2 (n/2+1) elements of 2 s_work bytes; real grid n^3 of s_grid)."""
+def stage_bytes(cfg, s_work, s_grid, kshape, rs):
+    """Algorithmic HBM bytes per launch of each stage: the data each stage
+    must read and write once.  Complex half grid n^2 (n/2+1) elements of
+    2 s_work bytes; real grid n^3 of s_grid; compact k-subgrid values
+    ks0 ks1 (ks2/2+1) of s_work; pruned inverse arrays rs n (n/2+1) and
+    rs^2 (n/2+1) complex; r subgrid rs^3 real."""
     n = cfg["n"]
     nh = n // 2 + 1
     half = n * n * nh * 2 * s_work
-    real = n ** 3
+    pk = kshape[0] * kshape[1] * (kshape[2] // 2 + 1) * s_work
+    q1 = rs * n * nh * 2 * s_work
+    q2 = rs * rs * nh * 2 * s_work
+    xs = rs ** 3 * s_work
     return {
-        "fft_z_fwd": real * s_grid + half,
-        "fft_y_fwd": 2 * half, "fft_x_fwd": 2 * half,
-        "fft_x_inv": 2 * half, "fft_y_inv": 2 * half,
-        "fft_z_inv": half + real * s_work,
-        "combine": 2.5 * half,  # interlace (read 2, write 1) and |F|^2 (read 1, write 1), averaged
-        "hist_k": ks * 2 * s_work,
-        "hist_r": rs * s_work,
+        "fft_z_fwd": n ** 3 * s_grid + half,
+        "fft_y_fwd": 2 * half,
+        "combine": 2 * half + pk + q1,  # fused x pass: read a, b; write pk, pruned q1
+        "hist_k": pk,
+        "fft_y_inv": q1 + q2,
+        "fft_z_inv": q2 + xs,
+        "hist_r": xs,
     }
 
 
@@ -177,9 +183,7 @@ def main():
                         and np.array_equal(f32["res"]["r_counts"], f64["res"]["r_counts"]))
 
     # Roofline of the dominant stage (HIP events on the correlator's stream).
-    ks = int(np.prod(f32["kshape"]))
-    rs = int(np.prod(f32["rshape"]))
-    sb = stage_bytes(cfg, 4, 4, ks, rs)
+    sb = stage_bytes(cfg, 4, 4, f32["kshape"], f32["rshape"][0])
     st = {k: v for k, v in f32["stages"].items() if v[1] > 0}
     dom = max(st, key=lambda k: st[k][0])
     avg_ms = st[dom][0] / st[dom][1]

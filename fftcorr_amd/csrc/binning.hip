@@ -1,0 +1,93 @@
+// Host construction of bin-sorted subgrid cells (see BinnedCells).
+#include <algorithm>
+#include <thread>
+
+#include "hist_device.h"
+
+namespace fc {
+
+void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
+                        const std::vector<double>& ay, const std::vector<double>& az,
+                        double binsize, int nbins) {
+  const int s0 = (int)ax.size(), s1 = (int)ay.size(), s2 = (int)az.size();
+  if ((uint64_t)s0 * s1 * s2 > 0xffffffffULL) fail(FC_ERR_INVALID, "subgrid too large");
+  // Bin of cell (i, j, k): floor((|r| - 0) / binsize) with
+  // |r| = sqrt((x*x + y*y) + z*z) in FP64 (correlator.h:129-135, :180-186;
+  // histogram_list.h:57).  Host x86-64 code without FMA, like the reference.
+  auto bin_of = [&](int i, int j, int k) {
+#pragma clang fp contract(off)
+    const double norm = std::sqrt(ax[i] * ax[i] + ay[j] * ay[j] + az[k] * az[k]);
+    const double f = std::floor((norm - 0.0) / binsize);
+    return (f >= 0.0 && f < (double)nbins) ? (int)f : -1;
+  };
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  nt = std::min<unsigned>(nt, (unsigned)s0);
+  // Pass 1: per-thread counts over fixed i-slabs.
+  std::vector<std::vector<int64_t>> cnt(nt, std::vector<int64_t>(nbins, 0));
+  auto slab = [&](unsigned t, int* lo, int* hi) {
+    *lo = (int)((int64_t)s0 * t / nt);
+    *hi = (int)((int64_t)s0 * (t + 1) / nt);
+  };
+  {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+      th.emplace_back([&, t] {
+        int lo, hi;
+        slab(t, &lo, &hi);
+        for (int i = lo; i < hi; ++i)
+          for (int j = 0; j < s1; ++j)
+            for (int k = 0; k < s2; ++k) {
+              const int b = bin_of(i, j, k);
+              if (b >= 0) ++cnt[t][b];
+            }
+      });
+    for (auto& x : th) x.join();
+  }
+  // Offsets: bin-major, then slab order, so each bin keeps raster order.
+  std::vector<int64_t> start(nbins + 1, 0);
+  std::vector<std::vector<int64_t>> off(nt, std::vector<int64_t>(nbins, 0));
+  int64_t pos = 0;
+  bc.h_counts.assign(nbins, 0);
+  for (int b = 0; b < nbins; ++b) {
+    start[b] = pos;
+    for (unsigned t = 0; t < nt; ++t) {
+      off[t][b] = pos;
+      pos += cnt[t][b];
+    }
+    bc.h_counts[b] = (int)(pos - start[b]);
+  }
+  start[nbins] = pos;
+  if (pos > 0x7fffffffLL) fail(FC_ERR_INVALID, "subgrid too large");
+  std::vector<uint32_t> cells(pos > 0 ? pos : 1);
+  {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+      th.emplace_back([&, t] {
+        int lo, hi;
+        slab(t, &lo, &hi);
+        std::vector<int64_t>& o = off[t];
+        for (int i = lo; i < hi; ++i)
+          for (int j = 0; j < s1; ++j)
+            for (int k = 0; k < s2; ++k) {
+              const int b = bin_of(i, j, k);
+              if (b >= 0) cells[o[b]++] = (uint32_t)(((int64_t)i * s1 + j) * s2 + k);
+            }
+      });
+    for (auto& x : th) x.join();
+  }
+  std::vector<int> start32(nbins + 1);
+  for (int b = 0; b <= nbins; ++b) start32[b] = (int)start[b];
+  bc.n = pos;
+  bc.nbins = nbins;
+  bc.cells.allocate(sizeof(uint32_t) * cells.size());
+  bc.start.allocate(sizeof(int) * (nbins + 1));
+  bc.counts.allocate(sizeof(int) * nbins);
+  FC_HIP(hipMemcpy(bc.cells.as<void>(), cells.data(), sizeof(uint32_t) * cells.size(),
+                   hipMemcpyHostToDevice));
+  FC_HIP(hipMemcpy(bc.start.as<void>(), start32.data(), sizeof(int) * (nbins + 1),
+                   hipMemcpyHostToDevice));
+  FC_HIP(hipMemcpy(bc.counts.as<void>(), bc.h_counts.data(), sizeof(int) * nbins,
+                   hipMemcpyHostToDevice));
+}
+
+}  // namespace fc

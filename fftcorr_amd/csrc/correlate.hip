@@ -1,15 +1,20 @@
 // Correlators (cc/correlate/correlator.h:18-505) on the device.
 //
-// PeriodicCorrelator (line of sight z):
-//   z-pass R2C of the grid (copy_into_padded_array fused into the load)
-//   -> y, x forward passes -> |F|^2 or F conj(G) -> k-space multipoles
-//   (extract_fft_submatrix_c2r + Y_l0(k) W^-1(k) + histogram, every l in one
-//   pass) -> x, y inverse passes -> z-pass C2R -> r-space multipoles
-//   (extract_submatrix + Y_l0(r) + histogram, every l in one pass).
-// Correlator (Yamamoto, observer at the survey origin):
-//   per (l, m): z-pass R2C with Y_lm(x) * delta fused into the load -> y, x
-//   -> conj-multiply with the stored FFT of grid 2 -> accumulate the k
-//   subgrid -> inverse -> accumulate the r subgrid; histograms once per l.
+// PeriodicCorrelator (line of sight z), per call:
+//   z pass R2C of the grid (copy_into_padded_array fused into the load),
+//   y pass; [the same for the half-cell-shifted grid when interlacing];
+//   one fused x pass: forward x FFT, interlacing combine, |F|^2 (or
+//     F conj(G)), the real part of every element the k subgrid reads
+//     written to a compact array, inverse x FFT, and only the x lags the r
+//     subgrid reads written back (pruned inverse);
+//   k-space multipoles from the compact array (extract_fft_submatrix_c2r +
+//     Y_l0(k) W^-1(k) + histogram, every l in one pass);
+//   pruned inverse y pass, pruned z C2R straight into the r subgrid;
+//   r-space multipoles (extract_submatrix + Y_l0(r) + histogram).
+// Correlator (Yamamoto, observer at the survey origin), per (l, m):
+//   z pass with Y_lm(x) * delta fused into the load -> y, x -> conj-multiply
+//   with the stored FFT of grid 2 -> accumulate the k subgrid -> inverse ->
+//   accumulate the r subgrid; histograms once per l.
 // Work arrays are FP64 (FC_FP64) or FP32 (FC_FP32); Y_lm, windows, bin
 // indices and every sum stay in FP64.
 #include <algorithm>
@@ -38,15 +43,18 @@ struct fc_corr {
   std::vector<double> rbins, kbins;
   std::vector<double> h_rx, h_kx, h_ky, h_kz;
   fc::DeviceBuffer rx, kx, ky, kz, wx, wy, wz;  // FP64 axes and per-axis window W^2
-  // FFT plan + work arrays.
+  fc::BinnedCells kbin, rbin;                   // subgrid cells sorted by bin
+  // FFT plan + work arrays (complex rows of plan->ldc).
   std::shared_ptr<fc::FftPlan> plan;
   fc::DeviceBuffer work, grid2_fft, work_b;
   bool have_grid2 = false;
+  // Periodic: compact k-subgrid values, pruned inverse stages, r subgrid.
+  fc::DeviceBuffer pk, q1, q2, xisub;
   fc::DeviceBuffer rsub, ksub;  // FP64 subgrids (Yamamoto accumulation)
+  fc::DeviceBuffer ph[3];       // interlacing phase per axis
   // Outputs.
   fc::DeviceBuffer r_counts, r_hist, k_counts, k_hist, zerolag;
   fc::HistScratch hs;
-  double h_zerolag = 0.0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
   // Per-stage profiling: event pairs recorded during the last call(s).
@@ -61,12 +69,16 @@ struct fc_corr {
 namespace fc {
 namespace {
 
+constexpr int kModeAuto = 0, kModeCross = 1, kModeInterlaced = 2;
+
 // ---- element-wise ----------------------------------------------------------
-// out *= conj(in) (array_ops::multiply_with_conjugation, array_ops.cc:139-163);
+// Complex rows of ldc (n2h used): out *= conj(in)
+// (array_ops::multiply_with_conjugation, array_ops.cc:139-163);
 // in == nullptr: out = |out|^2.
 template <typename T>
 __global__ void k_conj_mult(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ in,
                             int64_t n) {
+#pragma clang fp contract(off)
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const cplx<T> a = out[i];
@@ -80,34 +92,173 @@ __global__ void k_conj_mult(cplx<T>* __restrict__ out, const cplx<T>* __restrict
 }
 
 // Interlacing: F = (F_a + F_b exp(i pi (fx/n0 + fy/n1 + fz/n2))) / 2 with
-// signed frequency indices in the numpy.fft.fftfreq convention.
+// signed frequency indices in the numpy.fft.fftfreq convention;
+// ph0/ph1/ph2[f] = exp(i pi fsigned / n) per axis (host tables, FP64).
 template <typename T>
 __global__ void k_interlace(cplx<T>* __restrict__ fa, const cplx<T>* __restrict__ fb, int n0,
-                            int n1, int n2, int n2h) {
-  const int64_t n = (int64_t)n0 * n1 * n2h;
+                            int n1, int64_t ldc, const double2* __restrict__ ph0,
+                            const double2* __restrict__ ph1, const double2* __restrict__ ph2) {
+  const int64_t n = (int64_t)n0 * n1 * ldc;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i % n2h);
-    const int64_t r = i / n2h;
+    const int c = (int)(i % ldc);
+    const int64_t r = i / ldc;
     const int b = (int)(r % n1), a = (int)(r / n1);
-    const int fx = a < (n0 + 1) / 2 ? a : a - n0;
-    const int fy = b < (n1 + 1) / 2 ? b : b - n1;
-    const int fz = c < (n2 + 1) / 2 ? c : c - n2;
-    const double ph = M_PI * ((double)fx / n0 + (double)fy / n1 + (double)fz / n2);
-    double sn, cs;
-    sincos(ph, &sn, &cs);
+    const double2 p = cmul(cmul(ph0[a], ph1[b]), ph2[c]);
     const cplx<T> A = fa[i], B = fb[i];
-    const double bx = (double)B.x * cs - (double)B.y * sn;
-    const double by = (double)B.x * sn + (double)B.y * cs;
+    const double bx = (double)B.x * p.x - (double)B.y * p.y;
+    const double by = (double)B.x * p.y + (double)B.y * p.x;
     fa[i] = cplx<T>{T(0.5 * ((double)A.x + bx)), T(0.5 * ((double)A.y + by))};
   }
 }
 
+// Host complex128 [rows][n2h] -> device rows of ldc.
 template <typename Tdst>
-__global__ void k_from_c128(cplx<Tdst>* __restrict__ d, const double2* __restrict__ s, int64_t n) {
+__global__ void k_from_c128(cplx<Tdst>* __restrict__ d, const double2* __restrict__ s,
+                            int64_t rows, int n2h, int64_t ldc) {
+  const int64_t n = rows * n2h;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    d[i] = cplx<Tdst>{(Tdst)s[i].x, (Tdst)s[i].y};
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / n2h;
+    const int c = (int)(i - r * n2h);
+    d[r * ldc + c] = cplx<Tdst>{(Tdst)s[i].x, (Tdst)s[i].y};
+  }
+}
+
+// ---- k-subgrid compact layout ---------------------------------------------
+// pk[sx][sy][c] holds Re P at the element with signed offsets
+// (sx - ox, sy - oy) along x, y and kz index c in [0, oz]: every element the
+// k subgrid reads (directly or reflected, fft_grid.cc:215-257).  Slots along
+// an axis of length n for index e: e + o (if e <= o) and e - n + o (if >= 0);
+// both exist only when the subgrid was truncated to n + 1 cells.
+struct KSlots {
+  int ks0, ks1, ks2;
+  __device__ __forceinline__ int oz() const { return ks2 / 2; }
+  __device__ __forceinline__ int64_t at(int sx, int sy, int c) const {
+    return ((int64_t)sx * ks1 + sy) * (ks2 / 2 + 1) + c;
+  }
+};
+
+__device__ __forceinline__ int slot_a(int e, int o) { return e <= o ? e + o : -1; }
+__device__ __forceinline__ int slot_b(int e, int n, int o) {
+  return e - n + o >= 0 ? e - n + o : -1;
+}
+
+// Writes Re P of element (e, y, c) to every k-subgrid slot that reads it.
+template <typename T>
+__device__ __forceinline__ void put_pk(T* pk, const KSlots& ks, int n0, int n1, int e, int y,
+                                       int c, T v) {
+  if (c > ks.oz()) return;
+  const int ox = ks.ks0 / 2, oy = ks.ks1 / 2;
+  const int sx[2] = {slot_a(e, ox), slot_b(e, n0, ox)};
+  const int sy[2] = {slot_a(y, oy), slot_b(y, n1, oy)};
+  for (int a = 0; a < 2; ++a) {
+    if (sx[a] < 0 || (a == 1 && sx[1] == sx[0])) continue;
+    for (int b = 0; b < 2; ++b) {
+      if (sy[b] < 0 || (b == 1 && sy[1] == sy[0])) continue;
+      pk[ks.at(sx[a], sy[b], c)] = v;
+    }
+  }
+}
+
+// Fallback (non-fast x length): gather the compact array from the full grid.
+template <typename T>
+__global__ void k_gather_pk(T* __restrict__ pk, const cplx<T>* __restrict__ P, KSlots ks, int n0,
+                            int n1, int64_t ldc) {
+  const int oz = ks.ks2 / 2;
+  const int64_t n = (int64_t)ks.ks0 * ks.ks1 * (oz + 1);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % (oz + 1));
+    const int64_t t = i / (oz + 1);
+    const int sy = (int)(t % ks.ks1), sx = (int)(t / ks.ks1);
+    const int e = ((sx - ks.ks0 / 2) % n0 + n0) % n0;
+    const int y = ((sy - ks.ks1 / 2) % n1 + n1) % n1;
+    pk[i] = P[((int64_t)e * n1 + y) * ldc + c].x;
+  }
+}
+
+// ---- fused periodic x pass --------------------------------------------------
+// One workgroup per (y row, tile of tc kz columns); block = tc * (L / E).
+// A: grid a after its z and y passes; B: grid b likewise (interlaced) or the
+// fully transformed grid 2 (cross).  Writes pk and the pruned inverse-x
+// output q[sx][y][c] for x lags sx - rs/2, sx < rs.
+template <typename T, int L, int MODE>
+__global__ void __launch_bounds__(1024)
+    k_periodic_x(const cplx<T>* __restrict__ A, const cplx<T>* __restrict__ B, int n1, int n2h,
+                 int64_t ldc, int tilesC, int tc, const cplx<T>* __restrict__ tw,
+                 const double2* __restrict__ ph0, const double2* __restrict__ ph1,
+                 const double2* __restrict__ ph2, T* __restrict__ pk, KSlots ks,
+                 cplx<T>* __restrict__ q, int rs) {
+  using C = cplx<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  constexpr int E = FftCfg<L>::E;
+  constexpr int TCOL = L / E;
+  const int y = blockIdx.x / tilesC;
+  const int c0 = (blockIdx.x - y * tilesC) * tc;
+  const int ncol = min(tc, n2h - c0);
+  const int64_t sL = (int64_t)n1 * ldc;
+  const int64_t base = (int64_t)y * ldc + c0;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int c = tid % tc, t = tid / tc;
+  C rb[E];
+  // L * tc elements over tc * (L / E) threads: E loads per thread, issued
+  // before the LDS stores.
+  auto load_tile = [&](const C* __restrict__ src) {
+    C tmp[E];
+#pragma unroll
+    for (int qq = 0; qq < E; ++qq) {
+      const int idx = tid + qq * nthr;
+      const int e = idx / tc, cc = idx - e * tc;
+      tmp[qq] = cc < ncol ? src[base + e * sL + cc] : C{T(0), T(0)};
+    }
+#pragma unroll
+    for (int qq = 0; qq < E; ++qq) lds[tid + qq * nthr] = tmp[qq];
+  };
+  if (MODE == kModeInterlaced) {
+    load_tile(B);
+    __syncthreads();
+    fft_lds<T, L, -1>(lds, tw, c, t, LayColFast{tc});
+#pragma unroll
+    for (int qq = 0; qq < E; ++qq) rb[qq] = lds[(t + qq * TCOL) * tc + c];
+    __syncthreads();
+  }
+  load_tile(A);
+  __syncthreads();
+  fft_lds<T, L, -1>(lds, tw, c, t, LayColFast{tc});
+  {
+#pragma clang fp contract(off)
+    const int cg = c0 + c;
+#pragma unroll
+    for (int qq = 0; qq < E; ++qq) {
+      const int e = t + qq * TCOL;
+      const C fa = lds[e * tc + c];
+      C p;
+      if (MODE == kModeInterlaced) {
+        const double2 ph = cmul(cmul(ph0[e], ph1[y]), ph2[c < ncol ? cg : 0]);
+        const C fb = rb[qq];
+        const double fx = 0.5 * ((double)fa.x + ((double)fb.x * ph.x - (double)fb.y * ph.y));
+        const double fy = 0.5 * ((double)fa.y + ((double)fb.x * ph.y + (double)fb.y * ph.x));
+        p = C{T(fx * fx + fy * fy), T(0)};
+      } else if (MODE == kModeCross) {
+        const C g = c < ncol ? B[base + e * sL + c] : C{T(0), T(0)};
+        p = C{fa.x * g.x + fa.y * g.y, fa.y * g.x - fa.x * g.y};
+      } else {
+        p = C{fa.x * fa.x + fa.y * fa.y, T(0)};
+      }
+      lds[e * tc + c] = p;
+      if (c < ncol) put_pk<T>(pk, ks, L, n1, e, y, cg, p.x);
+    }
+  }
+  __syncthreads();
+  fft_lds<T, L, +1>(lds, tw, c, t, LayColFast{tc});
+  const int o = rs / 2;
+  for (int idx = tid; idx < rs * tc; idx += nthr) {
+    const int sx = idx / tc, cc = idx - sx * tc;
+    const int e = ((sx - o) % L + L) % L;
+    if (cc < ncol) q[((int64_t)sx * n1 + y) * ldc + c0 + cc] = lds[e * tc + cc];
+  }
 }
 
 // ---- z-pass loader with Y_lm(x) weighting (correlator.h:435-436) -----------
@@ -121,9 +272,11 @@ struct LoadYlm {
   }
 };
 
-// ---- subgrid index maps (cc/grid/fft_grid.cc:182-257) ----------------------
+// ---- full-grid subgrid index maps (Yamamoto; cc/grid/fft_grid.cc:182-257) --
 struct KMap {
-  int n0, n1, n2h, ks0, ks1, ks2;
+  int n0, n1;
+  int64_t ldc;
+  int ks0, ks1, ks2;
   // Half-complex element (a, b, c) feeding k-subgrid cell (i, j, k).
   __device__ __forceinline__ int64_t src(int i, int j, int k) const {
     const int ox = ks0 / 2, oy = ks1 / 2, oz = ks2 / 2;
@@ -131,14 +284,16 @@ struct KMap {
     const int jj = ((n1 - oy + j) % n1 + n1) % n1;
     if (k < oz) {  // negative kz: reflected (-kx, -ky, |kz|) row
       const int iin = (n0 - ii) % n0, jjn = (n1 - jj) % n1;
-      return ((int64_t)iin * n1 + jjn) * n2h + (oz - k);
+      return ((int64_t)iin * n1 + jjn) * ldc + (oz - k);
     }
-    return ((int64_t)ii * n1 + jj) * n2h + (k - oz);
+    return ((int64_t)ii * n1 + jj) * ldc + (k - oz);
   }
 };
 
 struct RMap {
-  int n0, n1, n2, ld, rs;  // ld: real row stride of the padded work grid
+  int n0, n1, n2;
+  int64_t ld;  // real row stride of the padded work grid
+  int rs;
   __device__ __forceinline__ int64_t src(int i, int j, int k) const {
     // Periodic image of lag (i - o, j - o, k - o); the positive modulo also
     // covers subgrids wider than the grid (rmax beyond half the box).
@@ -150,52 +305,48 @@ struct RMap {
   }
 };
 
-// Periodic k-space multipoles, every l at once (correlator.h:296-320).
+// Periodic k-space multipoles, every l at once (correlator.h:296-320), from
+// the compact array.
 template <typename T, int NV>
 struct PeriodicKVals {
-  const cplx<T>* P;
-  KMap map;
+  const T* pk;
+  KSlots ks;
   const double *kx, *ky, *kz, *wx, *wy, *wz;
   double c[NV];
-  double dk;
-  int nbins;
-  __device__ __forceinline__ void operator()(int64_t idx, int* bin, double* v) const {
+  __device__ __forceinline__ void operator()(int64_t cell, double* v) const {
 #pragma clang fp contract(off)
-    const int k = (int)(idx % map.ks2);
-    const int64_t t = idx / map.ks2;
-    const int j = (int)(t % map.ks1), i = (int)(t / map.ks1);
+    const int k = (int)(cell % ks.ks2);
+    const int64_t t = cell / ks.ks2;
+    const int j = (int)(t % ks.ks1), i = (int)(t / ks.ks1);
     const double x = kx[i], y = ky[j], z = kz[k];
-    *bin = hist_bin(norm3(x, y, z), 0.0, dk, nbins);
-    if (*bin < 0) return;
-    const double re = (double)P[map.src(i, j, k)].x;
+    const int oz = ks.oz();
+    const int64_t src = k >= oz ? ks.at(i, j, k - oz)
+                                : ks.at(ks.ks0 - 1 - i, ks.ks1 - 1 - j, oz - k);
+    const double re = (double)pk[src];
     const double invw = 1.0 / (wx[i] * wy[j] * wz[k]);
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = ylm_eval(2 * q, 0, x, y, z, c[q], invw) * re;
   }
 };
 
-// Periodic r-space multipoles, every l at once (correlator.h:336-355).
+// Periodic r-space multipoles, every l at once (correlator.h:336-355), from
+// the r subgrid xi[rs][rs][rs].
 template <typename T, int NV>
 struct PeriodicRVals {
   const T* xi;
-  RMap map;
+  int rs;
   const double* rx;
   double c[NV];
-  double dr;
-  int nbins;
   double* zerolag;
-  __device__ __forceinline__ void operator()(int64_t idx, int* bin, double* v) const {
+  __device__ __forceinline__ void operator()(int64_t cell, double* v) const {
 #pragma clang fp contract(off)
-    const int rs = map.rs;
-    const int k = (int)(idx % rs);
-    const int64_t t = idx / rs;
+    const int k = (int)(cell % rs);
+    const int64_t t = cell / rs;
     const int j = (int)(t % rs), i = (int)(t / rs);
     const double x = rx[i], y = rx[j], z = rx[k];
-    const double val = (double)xi[map.src(i, j, k)];
+    const double val = (double)xi[cell];
     const int o = rs / 2;
     if (i == o && j == o && k == o) *zerolag = ylm_eval(0, 0, x, y, z, c[0], 1.0) * val;
-    *bin = hist_bin(norm3(x, y, z), 0.0, dr, nbins);
-    if (*bin < 0) return;
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = ylm_eval(2 * q, 0, x, y, z, c[q], 1.0) * val;
   }
@@ -204,17 +355,7 @@ struct PeriodicRVals {
 // Histogram of an FP64 subgrid (one multipole row).
 struct SubgridVals {
   const double* sub;
-  const double *ax, *ay, *az;
-  int s1, s2;
-  double binsize;
-  int nbins;
-  __device__ __forceinline__ void operator()(int64_t idx, int* bin, double* v) const {
-    const int k = (int)(idx % s2);
-    const int64_t t = idx / s2;
-    const int j = (int)(t % s1), i = (int)(t / s1);
-    *bin = hist_bin(norm3(ax[i], ay[j], az[k]), 0.0, binsize, nbins);
-    v[0] = sub[idx];
-  }
+  __device__ __forceinline__ void operator()(int64_t cell, double* v) const { v[0] = sub[cell]; }
 };
 
 // Yamamoto accumulation into the k subgrid (correlator.h:453-466).
@@ -252,6 +393,22 @@ __global__ void k_acc_r(double* __restrict__ rsub, const T* __restrict__ xi, RMa
     const double y = ylm_eval(ell, m, rx[i], rx[j], rx[k], c, 1.0);
     rsub[idx] += y * (double)xi[map.src(i, j, k)];
   }
+}
+
+// Fill kernel (used instead of hipMemsetAsync so call sequences stay
+// capturable into HIP graphs).
+template <typename V>
+__global__ void k_fill(V* __restrict__ p, int64_t n, V v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+template <typename V>
+void fill(V* p, int64_t n, V v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill<V>, dim3(grid1d(n, 256, 4096)), dim3(256), 0, s, p, n, v);
+  FC_LAUNCH_CHECK();
 }
 
 // ---- host orchestration -----------------------------------------------------
@@ -363,6 +520,8 @@ void setup_geometry(fc_corr* c) {
     w[i].resize(c->kshape[i]);
     for (int k = 0; k < c->kshape[i]; ++k) w[i][k] = window_sq(c->window, (*axes[i])[k], c->cell);
   }
+  build_binned_cells(c->kbin, c->h_kx, c->h_ky, c->h_kz, c->dk, c->nkbins);
+  build_binned_cells(c->rbin, c->h_rx, c->h_rx, c->h_rx, c->dr, c->nrbins);
   upload(c->rx, c->h_rx);
   upload(c->kx, c->h_kx);
   upload(c->ky, c->h_ky);
@@ -378,11 +537,11 @@ void setup_geometry(fc_corr* c) {
 }
 
 void reset_hists(fc_corr* c, hipStream_t s) {
-  FC_HIP(hipMemsetAsync(c->r_counts.as<void>(), 0, sizeof(int) * c->nell * c->nrbins, s));
-  FC_HIP(hipMemsetAsync(c->r_hist.as<void>(), 0, sizeof(double) * c->nell * c->nrbins, s));
-  FC_HIP(hipMemsetAsync(c->k_counts.as<void>(), 0, sizeof(int) * c->nell * c->nkbins, s));
-  FC_HIP(hipMemsetAsync(c->k_hist.as<void>(), 0, sizeof(double) * c->nell * c->nkbins, s));
-  FC_HIP(hipMemsetAsync(c->zerolag.as<void>(), 0, sizeof(double), s));
+  fill<int>(c->r_counts.as<int>(), (int64_t)c->nell * c->nrbins, 0, s);
+  fill<double>(c->r_hist.as<double>(), (int64_t)c->nell * c->nrbins, 0.0, s);
+  fill<int>(c->k_counts.as<int>(), (int64_t)c->nell * c->nkbins, 0, s);
+  fill<double>(c->k_hist.as<double>(), (int64_t)c->nell * c->nkbins, 0.0, s);
+  fill<double>(c->zerolag.as<double>(), 1, 0.0, s);
 }
 
 // Orders the correlator's stream after all work queued on the grid's stream.
@@ -403,69 +562,139 @@ void check_grid_shape(fc_corr* c, fc_grid* g) {
     fail(FC_ERR_INVALID, "grid and correlator live on different devices");
 }
 
-template <typename T>
-cplx<T>* work_of(fc_corr* c) {
-  return c->work.as<cplx<T>>();
-}
-
-// Forward 3D FFT of a device grid into dst (complex half layout).
+// z and y forward passes of a device grid into dst (complex rows of ldc).
 template <typename T, class Loader>
-void forward_fft(fc_corr* c, fc_grid* g, cplx<T>* dst, Loader ld) {
+void forward_zy(fc_corr* c, fc_grid* g, cplx<T>* dst, Loader ld) {
   const FftPlan& p = *c->plan;
   hipStream_t s = c->ctx->stream;
-  const int64_t ldi = p.n[2];
-  const int64_t n2h = p.n2h;
   {
     Stage st(c, FC_STAGE_FFT_Z_FWD);
     if (g->prec == FC_FP64)
-      launch_rows_r2c<T, double>(p, g->data.as<double>(), ldi, dst, n2h, ld, s);
+      launch_rows_r2c<T, double>(p, g->data.as<double>(), p.n[2], dst, p.ldc, ld, s);
     else
-      launch_rows_r2c<T, float>(p, g->data.as<float>(), ldi, dst, n2h, ld, s);
+      launch_rows_r2c<T, float>(p, g->data.as<float>(), p.n[2], dst, p.ldc, ld, s);
   }
-  {
-    Stage st(c, FC_STAGE_FFT_Y_FWD);
-    launch_cols<T, -1>(p.n[1], dst, (int64_t)p.n[1] * n2h, p.n[0], n2h, (int)n2h, p.tile[1],
-                       p.tw1.as<cplx<T>>(), s);
-  }
-  {
-    Stage st(c, FC_STAGE_FFT_X_FWD);
-    launch_cols<T, -1>(p.n[0], dst, 0, 1, (int64_t)p.n[1] * n2h, (int)(p.n[1] * n2h),
-                       p.tile[0], p.tw0.as<cplx<T>>(), s);
-  }
+  Stage st(c, FC_STAGE_FFT_Y_FWD);
+  launch_cols<T, -1>(p.n[1], y_view<T>(p, dst), p.tile[1], p.tw1.as<cplx<T>>(), s);
 }
 
-// Inverse 3D FFT in place: complex half layout -> padded real rows.
+template <typename T>
+void forward_x(fc_corr* c, cplx<T>* dst) {
+  const FftPlan& p = *c->plan;
+  Stage st(c, FC_STAGE_FFT_X_FWD);
+  launch_cols<T, -1>(p.n[0], x_view<T>(p, dst), p.tile[0], p.tw0.as<cplx<T>>(), c->ctx->stream);
+}
+
+// Full forward 3D FFT.
+template <typename T, class Loader>
+void forward_fft(fc_corr* c, fc_grid* g, cplx<T>* dst, Loader ld) {
+  forward_zy<T>(c, g, dst, ld);
+  forward_x<T>(c, dst);
+}
+
+// Full inverse 3D FFT in place: complex rows -> padded real rows (2 ldc).
 template <typename T>
 void inverse_fft(fc_corr* c, cplx<T>* data) {
   const FftPlan& p = *c->plan;
   hipStream_t s = c->ctx->stream;
-  const int64_t n2h = p.n2h;
   {
     Stage st(c, FC_STAGE_FFT_X_INV);
-    launch_cols<T, +1>(p.n[0], data, 0, 1, (int64_t)p.n[1] * n2h, (int)(p.n[1] * n2h),
-                       p.tile[0], p.tw0.as<cplx<T>>(), s);
+    launch_cols<T, +1>(p.n[0], x_view<T>(p, data), p.tile[0], p.tw0.as<cplx<T>>(), s);
   }
   {
     Stage st(c, FC_STAGE_FFT_Y_INV);
-    launch_cols<T, +1>(p.n[1], data, (int64_t)p.n[1] * n2h, p.n[0], n2h, (int)n2h, p.tile[1],
-                       p.tw1.as<cplx<T>>(), s);
+    launch_cols<T, +1>(p.n[1], y_view<T>(p, data), p.tile[1], p.tw1.as<cplx<T>>(), s);
   }
-  {
-    Stage st(c, FC_STAGE_FFT_Z_INV);
-    launch_rows_c2r<T, T>(p, data, n2h, reinterpret_cast<T*>(data), 2 * n2h, s);
+  Stage st(c, FC_STAGE_FFT_Z_INV);
+  launch_rows_c2r<T, T>(p, data, p.ldc, reinterpret_cast<T*>(data), 2 * (int64_t)p.ldc, s);
+}
+
+// exp(i pi f / n) for the signed frequency f of every index along each axis
+// (numpy.fft.fftfreq convention), for interlacing.
+void ensure_phase_tables(fc_corr* c) {
+  if (c->ph[0].bytes()) return;
+  const int len[3] = {c->shape[0], c->shape[1], c->plan->ldc};
+  for (int d = 0; d < 3; ++d) {
+    const int n = c->shape[d];
+    std::vector<double2> h(len[d]);
+    for (int a = 0; a < len[d]; ++a) {
+      const int f = a < (n + 1) / 2 ? a : a - n;
+      const long double ph = 3.141592653589793238462643383279502884L * f / n;
+      h[a] = make_double2((double)cosl(ph), (double)sinl(ph));
+    }
+    c->ph[d].allocate(sizeof(double2) * len[d]);
+    FC_HIP(hipMemcpy(c->ph[d].as<void>(), h.data(), sizeof(double2) * len[d],
+                     hipMemcpyHostToDevice));
   }
 }
 
+KSlots kslots(const fc_corr* c) { return KSlots{c->kshape[0], c->kshape[1], c->kshape[2]}; }
+
+template <typename T>
+void ensure_periodic_buffers(fc_corr* c) {
+  const FftPlan& p = *c->plan;
+  const int rs = c->rshape;
+  const int oz = c->kshape[2] / 2;
+  c->pk.reserve(sizeof(T) * (size_t)c->kshape[0] * c->kshape[1] * (oz + 1));
+  c->q1.reserve(sizeof(cplx<T>) * (size_t)rs * p.n[1] * p.ldc);
+  c->q2.reserve(sizeof(cplx<T>) * (size_t)rs * rs * p.ldc);
+  c->xisub.reserve(sizeof(T) * (size_t)rs * rs * rs);
+}
+
+template <typename T, int MODE>
+void launch_periodic_x(fc_corr* c, cplx<T>* a, const cplx<T>* b) {
+  const FftPlan& p = *c->plan;
+  const int tc = p.tile[0];
+  const int tilesC = (p.n2h + tc - 1) / tc;
+  const size_t cs = sizeof(cplx<T>);
+  const double2* ph[3] = {c->ph[0].as<double2>(), c->ph[1].as<double2>(),
+                          c->ph[2].as<double2>()};
+#define FC_PX_CASE(Lv)                                                                      \
+  case Lv: {                                                                                \
+    const int thr = tc * (Lv / FftCfg<Lv>::E);                                              \
+    hipLaunchKernelGGL((k_periodic_x<T, Lv, MODE>), dim3((unsigned)(p.n[1] * tilesC)),     \
+                       dim3(thr), (size_t)Lv * tc * cs, c->ctx->stream, a, b, p.n[1], p.n2h, \
+                       (int64_t)p.ldc, tilesC, tc, p.tw0.as<cplx<T>>(), ph[0], ph[1], ph[2],  \
+                       c->pk.as<T>(), kslots(c), c->q1.as<cplx<T>>(), c->rshape);          \
+    break;                                                                                  \
+  }
+  switch (p.n[0]) {
+    FC_PX_CASE(8)
+    FC_PX_CASE(16)
+    FC_PX_CASE(32)
+    FC_PX_CASE(64)
+    FC_PX_CASE(128)
+    FC_PX_CASE(256)
+    FC_PX_CASE(512)
+    FC_PX_CASE(1024)
+    FC_PX_CASE(2048)
+    FC_PX_CASE(4096)
+    FC_PX_CASE(24)
+    FC_PX_CASE(48)
+    FC_PX_CASE(96)
+    FC_PX_CASE(192)
+    FC_PX_CASE(384)
+    FC_PX_CASE(768)
+    FC_PX_CASE(1536)
+    FC_PX_CASE(3072)
+    default:
+      fail(FC_ERR_RUNTIME, "internal: no fused x kernel for this length");
+  }
+#undef FC_PX_CASE
+  FC_LAUNCH_CHECK();
+}
+
 template <typename T, int NV>
-void periodic_hists(fc_corr* c, const cplx<T>* P) {
+void periodic_tail(fc_corr* c) {
   hipStream_t s = c->ctx->stream;
   const FftPlan& p = *c->plan;
+  const int rs = c->rshape, o = rs / 2;
   const uint64_t ncells = (uint64_t)c->shape[0] * c->shape[1] * c->shape[2];
   // correlator.h:294 and :310-313: k_rescale = cell^3 / N.
   const double k_rescale = c->cell * c->cell * c->cell / ncells;
   PeriodicKVals<T, NV> kv;
-  kv.P = P;
-  kv.map = KMap{p.n[0], p.n[1], p.n2h, c->kshape[0], c->kshape[1], c->kshape[2]};
+  kv.pk = c->pk.as<T>();
+  kv.ks = kslots(c);
   kv.kx = c->kx.as<double>();
   kv.ky = c->ky.as<double>();
   kv.kz = c->kz.as<double>();
@@ -474,76 +703,106 @@ void periodic_hists(fc_corr* c, const cplx<T>* P) {
   kv.wz = c->wz.as<double>();
   for (int q = 0; q < NV; ++q)
     kv.c[q] = ylm_coeff(2 * q, 0, std::sqrt((4.0 * M_PI) * (2 * (2 * q) + 1)) * k_rescale);
-  kv.dk = c->dk;
-  kv.nbins = c->nkbins;
-  const int64_t nk = (int64_t)c->kshape[0] * c->kshape[1] * c->kshape[2];
   {
     Stage st(c, FC_STAGE_HIST_K);
-    run_hist<NV>(kv, nk, c->nkbins, c->hs, c->k_hist.as<double>(), c->k_counts.as<int>(), 0,
-                 0, NV, s);
+    run_hist_sorted<NV>(kv, c->kbin, c->hs, c->k_hist.as<double>(), c->k_counts.as<int>(), 0, 0,
+                        NV, s);
   }
-  inverse_fft<T>(c, const_cast<cplx<T>*>(P));
+  // Pruned inverse y pass: q1[sx][y][c] -> q2[sx][sy][c], only the rs lags.
+  {
+    Stage st(c, FC_STAGE_FFT_Y_INV);
+    const int64_t ldc = p.ldc;
+    ColView<T> v{c->q1.as<cplx<T>>(), (int64_t)p.n[1] * ldc, ldc, 0, c->q2.as<cplx<T>>(),
+                 (int64_t)rs * ldc, ldc, 0, rs, 1, p.n2h, rs, o};
+    launch_cols<T, +1>(p.n[1], v, p.tile[1], p.tw1.as<cplx<T>>(), s);
+  }
+  // Pruned z C2R: rows of q2 -> r subgrid rows xi[sx][sy][sz].
+  {
+    Stage st(c, FC_STAGE_FFT_Z_INV);
+    launch_rows_c2r<T, T>(p, c->q2.as<cplx<T>>(), p.ldc, c->xisub.as<T>(), rs, s,
+                          (int64_t)rs * rs, rs, o);
+  }
   // correlator.h:336: r_rescale = 1 / N / N.
   const double r_rescale = (1.0 / ncells / ncells);
   PeriodicRVals<T, NV> rv;
-  rv.xi = reinterpret_cast<const T*>(P);
-  rv.map = RMap{p.n[0], p.n[1], p.n[2], 2 * p.n2h, c->rshape};
+  rv.xi = c->xisub.as<T>();
+  rv.rs = rs;
   rv.rx = c->rx.as<double>();
   for (int q = 0; q < NV; ++q)
     rv.c[q] = ylm_coeff(2 * q, 0, std::sqrt((4.0 * M_PI) * (2 * (2 * q) + 1)) * r_rescale);
-  rv.dr = c->dr;
-  rv.nbins = c->nrbins;
   rv.zerolag = c->zerolag.as<double>();
-  const int64_t nr = (int64_t)c->rshape * c->rshape * c->rshape;
   Stage st(c, FC_STAGE_HIST_R);
-  run_hist<NV>(rv, nr, c->nrbins, c->hs, c->r_hist.as<double>(), c->r_counts.as<int>(), 0, 0,
-               NV, s);
+  run_hist_sorted<NV>(rv, c->rbin, c->hs, c->r_hist.as<double>(), c->r_counts.as<int>(), 0, 0,
+                      NV, s);
 }
 
 template <typename T>
-void periodic_from_power(fc_corr* c, const cplx<T>* P) {
-  switch (c->nell) {
-    case 1:
-      periodic_hists<T, 1>(c, P);
-      break;
-    case 2:
-      periodic_hists<T, 2>(c, P);
-      break;
-    default:
-      periodic_hists<T, 3>(c, P);
-  }
-}
-
-template <typename T>
-void periodic_correlate(fc_corr* c, fc_grid* g1, bool cross, fc_grid* g_interlaced) {
+void periodic_correlate(fc_corr* c, fc_grid* g1, int mode, fc_grid* gb) {
   hipStream_t s = c->ctx->stream;
   const FftPlan& p = *c->plan;
-  const int64_t nc = p.nrows() * p.n2h;
-  cplx<T>* w = work_of<T>(c);
-  forward_fft<T>(c, g1, w, LoadCopy{});
-  if (g_interlaced) {
-    c->work_b.reserve(p.nrows() * p.n2h * sizeof(cplx<T>));
-    forward_fft<T>(c, g_interlaced, c->work_b.as<cplx<T>>(), LoadCopy{});
-    Stage st(c, FC_STAGE_COMBINE);
-    hipLaunchKernelGGL(k_interlace<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, w,
-                       c->work_b.as<cplx<T>>(), p.n[0], p.n[1], p.n[2], p.n2h);
-    FC_LAUNCH_CHECK();
+  const int64_t nc = p.nrows() * p.ldc;
+  cplx<T>* w = c->work.as<cplx<T>>();
+  ensure_periodic_buffers<T>(c);
+  ensure_phase_tables(c);
+  forward_zy<T>(c, g1, w, LoadCopy{});
+  const cplx<T>* b = nullptr;
+  if (mode == kModeInterlaced) {
+    c->work_b.reserve(p.complex_bytes());
+    forward_zy<T>(c, gb, c->work_b.as<cplx<T>>(), LoadCopy{});
+    b = c->work_b.as<cplx<T>>();
+  } else if (mode == kModeCross) {
+    b = c->grid2_fft.as<cplx<T>>();
   }
-  {
+  if (p.fast[0]) {
     Stage st(c, FC_STAGE_COMBINE);
+    if (mode == kModeInterlaced)
+      launch_periodic_x<T, kModeInterlaced>(c, w, b);
+    else if (mode == kModeCross)
+      launch_periodic_x<T, kModeCross>(c, w, b);
+    else
+      launch_periodic_x<T, kModeAuto>(c, w, b);
+  } else {
+    // Unusual x length: separate x pass, combine, gather, pruned inverse x.
+    forward_x<T>(c, w);
+    if (mode == kModeInterlaced) forward_x<T>(c, c->work_b.as<cplx<T>>());
+    Stage st(c, FC_STAGE_COMBINE);
+    if (mode == kModeInterlaced) {
+      hipLaunchKernelGGL(k_interlace<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, w, b,
+                         p.n[0], p.n[1], (int64_t)p.ldc, c->ph[0].as<double2>(),
+                         c->ph[1].as<double2>(), c->ph[2].as<double2>());
+      FC_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_conj_mult<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, w,
-                       cross ? c->grid2_fft.as<cplx<T>>() : nullptr, nc);
+                       mode == kModeCross ? b : nullptr, nc);
     FC_LAUNCH_CHECK();
+    const KSlots ks = kslots(c);
+    const int64_t npk = (int64_t)ks.ks0 * ks.ks1 * (ks.ks2 / 2 + 1);
+    hipLaunchKernelGGL(k_gather_pk<T>, dim3(grid1d(npk, 256, 65536)), dim3(256), 0, s,
+                       c->pk.as<T>(), w, ks, p.n[0], p.n[1], (int64_t)p.ldc);
+    FC_LAUNCH_CHECK();
+    const int64_t ldc = p.ldc;
+    ColView<T> v{w, 0, (int64_t)p.n[1] * ldc, ldc, c->q1.as<cplx<T>>(), 0,
+                 (int64_t)p.n[1] * ldc, ldc, 1, p.n[1], p.n2h, c->rshape, c->rshape / 2};
+    launch_cols<T, +1>(p.n[0], v, p.tile[0], p.tw0.as<cplx<T>>(), s);
   }
-  periodic_from_power<T>(c, w);
+  switch (c->nell) {
+    case 1:
+      periodic_tail<T, 1>(c);
+      break;
+    case 2:
+      periodic_tail<T, 2>(c);
+      break;
+    default:
+      periodic_tail<T, 3>(c);
+  }
 }
 
 template <typename T>
 void yamamoto_correlate(fc_corr* c, fc_grid* g1) {
   hipStream_t s = c->ctx->stream;
   const FftPlan& p = *c->plan;
-  const int64_t nc = p.nrows() * p.n2h;
-  cplx<T>* w = work_of<T>(c);
+  const int64_t nc = p.nrows() * p.ldc;
+  cplx<T>* w = c->work.as<cplx<T>>();
   const uint64_t ncells = (uint64_t)c->shape[0] * c->shape[1] * c->shape[2];
   // Observer at the survey origin; cell centres 0.5 - observer + i
   // (correlator.h:381-402).
@@ -553,11 +812,11 @@ void yamamoto_correlate(fc_corr* c, fc_grid* g1) {
   const int64_t nr = (int64_t)c->rshape * c->rshape * c->rshape;
   c->rsub.reserve(sizeof(double) * nr);
   c->ksub.reserve(sizeof(double) * nk);
-  KMap km{p.n[0], p.n[1], p.n2h, c->kshape[0], c->kshape[1], c->kshape[2]};
-  RMap rm{p.n[0], p.n[1], p.n[2], 2 * p.n2h, c->rshape};
+  KMap km{p.n[0], p.n[1], (int64_t)p.ldc, c->kshape[0], c->kshape[1], c->kshape[2]};
+  RMap rm{p.n[0], p.n[1], p.n[2], 2 * (int64_t)p.ldc, c->rshape};
   for (int ell = 0; ell <= c->maxell; ell += 2) {
-    FC_HIP(hipMemsetAsync(c->rsub.as<void>(), 0, sizeof(double) * nr, s));
-    FC_HIP(hipMemsetAsync(c->ksub.as<void>(), 0, sizeof(double) * nk, s));
+    fill<double>(c->rsub.as<double>(), nr, 0.0, s);
+    fill<double>(c->ksub.as<double>(), nk, 0.0, s);
     for (int m = -ell; m <= ell; ++m) {
       LoadYlm ld{ell, m, p.n[1], ylm_coeff(ell, m, 1.0), start[0], start[1], start[2]};
       forward_fft<T>(c, g1, w, ld);
@@ -584,19 +843,17 @@ void yamamoto_correlate(fc_corr* c, fc_grid* g1) {
         FC_LAUNCH_CHECK();
       }
     }
-    SubgridVals rv{c->rsub.as<double>(), c->rx.as<double>(), c->rx.as<double>(),
-                   c->rx.as<double>(), c->rshape, c->rshape, c->dr, c->nrbins};
-    SubgridVals kv{c->ksub.as<double>(), c->kx.as<double>(), c->ky.as<double>(),
-                   c->kz.as<double>(), c->kshape[1], c->kshape[2], c->dk, c->nkbins};
     {
       Stage st(c, FC_STAGE_HIST_R);
-      run_hist<1>(rv, nr, c->nrbins, c->hs, c->r_hist.as<double>(), c->r_counts.as<int>(),
-                  ell / 2, ell / 2, ell / 2 + 1, s);
+      run_hist_sorted<1>(SubgridVals{c->rsub.as<double>()}, c->rbin, c->hs,
+                         c->r_hist.as<double>(), c->r_counts.as<int>(), ell / 2, ell / 2,
+                         ell / 2 + 1, s);
     }
     {
       Stage st(c, FC_STAGE_HIST_K);
-      run_hist<1>(kv, nk, c->nkbins, c->hs, c->k_hist.as<double>(), c->k_counts.as<int>(),
-                  ell / 2, ell / 2, ell / 2 + 1, s);
+      run_hist_sorted<1>(SubgridVals{c->ksub.as<double>()}, c->kbin, c->hs,
+                         c->k_hist.as<double>(), c->k_counts.as<int>(), ell / 2, ell / 2,
+                         ell / 2 + 1, s);
     }
     if (ell == 0) {
       const int o = c->rshape / 2;
@@ -607,9 +864,7 @@ void yamamoto_correlate(fc_corr* c, fc_grid* g1) {
   }
 }
 
-void ensure_work(fc_corr* c) {
-  c->work.reserve(c->plan->complex_bytes());
-}
+void ensure_work(fc_corr* c) { c->work.reserve(c->plan->complex_bytes()); }
 
 void set_grid2_device(fc_corr* c, fc_grid* g2) {
   ensure_work(c);
@@ -621,7 +876,7 @@ void set_grid2_device(fc_corr* c, fc_grid* g2) {
   c->have_grid2 = true;
 }
 
-void correlate(fc_corr* c, fc_grid* g1, fc_grid* gb, int mode /*0 auto,1 cross,2 interlaced*/) {
+void correlate(fc_corr* c, fc_grid* g1, fc_grid* gb, int mode) {
   DeviceGuard dg(c->ctx->device);
   hipStream_t s = c->ctx->stream;
   check_grid_shape(c, g1);
@@ -635,19 +890,21 @@ void correlate(fc_corr* c, fc_grid* g1, fc_grid* gb, int mode /*0 auto,1 cross,2
   FC_HIP(hipEventRecord(c->ev0, s));
   reset_hists(c, s);
   if (c->kind == FC_YAMAMOTO) {
-    if (mode == 0) set_grid2_device(c, g1);  // Correlator::autocorrelate = cross(grid, grid)
+    if (mode == kModeInterlaced)
+      fail(FC_ERR_INVALID, "interlacing applies to the periodic correlator");
+    if (mode == kModeAuto) set_grid2_device(c, g1);  // Correlator::autocorrelate = cross(grid, grid)
     if (!c->have_grid2) fail(FC_ERR_RUNTIME, "grid2 FFT not set: call set_grid2 first");
-    if (mode == 2) fail(FC_ERR_INVALID, "interlacing applies to the periodic correlator");
     if (c->prec == FC_FP64)
       yamamoto_correlate<double>(c, g1);
     else
       yamamoto_correlate<float>(c, g1);
   } else {
-    if (mode == 1 && !c->have_grid2) fail(FC_ERR_RUNTIME, "grid2 FFT not set: call set_grid2 first");
+    if (mode == kModeCross && !c->have_grid2)
+      fail(FC_ERR_RUNTIME, "grid2 FFT not set: call set_grid2 first");
     if (c->prec == FC_FP64)
-      periodic_correlate<double>(c, g1, mode == 1, mode == 2 ? gb : nullptr);
+      periodic_correlate<double>(c, g1, mode, gb);
     else
-      periodic_correlate<float>(c, g1, mode == 1, mode == 2 ? gb : nullptr);
+      periodic_correlate<float>(c, g1, mode, gb);
   }
   FC_HIP(hipEventRecord(c->ev1, s));
   c->timed = true;
@@ -741,17 +998,22 @@ int fc_corr_set_grid2_fft(fc_corr* c, const double* host_complex) {
     if (!host_complex) fc::fail(FC_ERR_INVALID, "null array");
     fc::DeviceGuard dg(c->ctx->device);
     hipStream_t s = c->ctx->stream;
-    const int64_t n = c->plan->nrows() * c->plan->n2h;
-    c->grid2_fft.reserve(c->plan->complex_bytes());
+    const fc::FftPlan& p = *c->plan;
+    const int64_t rows = p.nrows();
+    const int64_t n = rows * p.n2h;
+    c->grid2_fft.reserve(p.complex_bytes());
     fc::DeviceBuffer tmp(sizeof(double2) * n);
     FC_HIP(hipMemcpyAsync(tmp.as<void>(), host_complex, sizeof(double2) * n,
                           hipMemcpyHostToDevice, s));
     if (c->prec == FC_FP64)
-      FC_HIP(hipMemcpyAsync(c->grid2_fft.as<void>(), tmp.as<void>(), sizeof(double2) * n,
-                            hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL(fc::k_from_c128<double>, dim3(fc::grid1d(n, 256, 65536)), dim3(256), 0,
+                         s, c->grid2_fft.as<double2>(), tmp.as<double2>(), rows, p.n2h,
+                         (int64_t)p.ldc);
     else
       hipLaunchKernelGGL(fc::k_from_c128<float>, dim3(fc::grid1d(n, 256, 65536)), dim3(256), 0,
-                         s, c->grid2_fft.as<float2>(), tmp.as<double2>(), n);
+                         s, c->grid2_fft.as<float2>(), tmp.as<double2>(), rows, p.n2h,
+                         (int64_t)p.ldc);
+    FC_LAUNCH_CHECK();
     FC_HIP(hipStreamSynchronize(s));
     c->have_grid2 = true;
   });
@@ -760,14 +1022,14 @@ int fc_corr_set_grid2_fft(fc_corr* c, const double* host_complex) {
 int fc_corr_autocorrelate(fc_corr* c, fc_grid* grid) {
   return fc::guard([&] {
     if (!c) fc::fail(FC_ERR_INVALID, "null correlator");
-    fc::correlate(c, grid, nullptr, 0);
+    fc::correlate(c, grid, nullptr, fc::kModeAuto);
   });
 }
 
 int fc_corr_cross_correlate(fc_corr* c, fc_grid* grid1) {
   return fc::guard([&] {
     if (!c) fc::fail(FC_ERR_INVALID, "null correlator");
-    fc::correlate(c, grid1, nullptr, 1);
+    fc::correlate(c, grid1, nullptr, fc::kModeCross);
   });
 }
 
@@ -775,7 +1037,7 @@ int fc_corr_autocorrelate_interlaced(fc_corr* c, fc_grid* a, fc_grid* b) {
   return fc::guard([&] {
     if (!c) fc::fail(FC_ERR_INVALID, "null correlator");
     if (!b) fc::fail(FC_ERR_INVALID, "null interlaced grid");
-    fc::correlate(c, a, b, 2);
+    fc::correlate(c, a, b, fc::kModeInterlaced);
   });
 }
 

@@ -87,6 +87,10 @@ std::shared_ptr<FftPlan> get_fft_plan(int device, const int shape[3], int prec) 
   p->n2h = shape[2] / 2 + 1;
   p->prec = prec;
   const size_t cs = p->csize();
+  {
+    const int align = (int)(128 / cs);
+    p->ldc = (p->n2h + align - 1) / align * align;
+  }
   p->fast[0] = fft_len_fast(shape[0]);
   p->fast[1] = fft_len_fast(shape[1]);
   p->fast[2] = (shape[2] % 2 == 0) && fft_len_fast(shape[2] / 2) && shape[2] / 2 <= 2048;
@@ -131,22 +135,23 @@ void fft_roundtrip_r2c(fc_ctx* ctx, const int shape[3], const double* real_in,
                        double* complex_out) {
   auto p = get_fft_plan(ctx->device, shape, std::is_same<T, double>::value ? FC_FP64 : FC_FP32);
   const int64_t nrows = p->nrows();
-  const int n2 = shape[2], n2h = p->n2h;
+  const int n2 = shape[2], n2h = p->n2h, ldc = p->ldc;
   DeviceBuffer din(sizeof(double) * nrows * n2);
-  DeviceBuffer work(sizeof(cplx<T>) * nrows * n2h);
+  DeviceBuffer work(sizeof(cplx<T>) * nrows * ldc);
   FC_HIP(hipMemcpyAsync(din.as<void>(), real_in, sizeof(double) * nrows * n2,
                         hipMemcpyHostToDevice, ctx->stream));
-  launch_rows_r2c<T, double>(*p, din.as<double>(), n2, work.as<cplx<T>>(), n2h, LoadCopy{},
+  launch_rows_r2c<T, double>(*p, din.as<double>(), n2, work.as<cplx<T>>(), ldc, LoadCopy{},
                              ctx->stream);
   fft_yx_forward<T>(*p, work.as<cplx<T>>(), ctx->stream);
-  std::vector<cplx<T>> h(nrows * n2h);
+  std::vector<cplx<T>> h(nrows * ldc);
   FC_HIP(hipMemcpyAsync(h.data(), work.as<void>(), sizeof(cplx<T>) * h.size(),
                         hipMemcpyDeviceToHost, ctx->stream));
   FC_HIP(hipStreamSynchronize(ctx->stream));
-  for (size_t i = 0; i < h.size(); ++i) {
-    complex_out[2 * i] = h[i].x;
-    complex_out[2 * i + 1] = h[i].y;
-  }
+  for (int64_t r = 0; r < nrows; ++r)
+    for (int k = 0; k < n2h; ++k) {
+      complex_out[2 * (r * n2h + k)] = h[r * ldc + k].x;
+      complex_out[2 * (r * n2h + k) + 1] = h[r * ldc + k].y;
+    }
 }
 
 template <typename T>
@@ -154,18 +159,19 @@ void fft_roundtrip_c2r(fc_ctx* ctx, const int shape[3], const double* complex_in
                        double* real_out) {
   auto p = get_fft_plan(ctx->device, shape, std::is_same<T, double>::value ? FC_FP64 : FC_FP32);
   const int64_t nrows = p->nrows();
-  const int n2 = shape[2], n2h = p->n2h;
-  std::vector<cplx<T>> h(nrows * n2h);
-  for (size_t i = 0; i < h.size(); ++i) {
-    h[i].x = (T)complex_in[2 * i];
-    h[i].y = (T)complex_in[2 * i + 1];
-  }
-  DeviceBuffer work(sizeof(cplx<T>) * nrows * n2h);
+  const int n2 = shape[2], n2h = p->n2h, ldc = p->ldc;
+  std::vector<cplx<T>> h(nrows * ldc, cplx<T>{T(0), T(0)});
+  for (int64_t r = 0; r < nrows; ++r)
+    for (int k = 0; k < n2h; ++k) {
+      h[r * ldc + k].x = (T)complex_in[2 * (r * n2h + k)];
+      h[r * ldc + k].y = (T)complex_in[2 * (r * n2h + k) + 1];
+    }
+  DeviceBuffer work(sizeof(cplx<T>) * nrows * ldc);
   DeviceBuffer dout(sizeof(double) * nrows * n2);
   FC_HIP(hipMemcpyAsync(work.as<void>(), h.data(), sizeof(cplx<T>) * h.size(),
                         hipMemcpyHostToDevice, ctx->stream));
   fft_xy_inverse<T>(*p, work.as<cplx<T>>(), ctx->stream);
-  launch_rows_c2r<T, double>(*p, work.as<cplx<T>>(), n2h, dout.as<double>(), n2, ctx->stream);
+  launch_rows_c2r<T, double>(*p, work.as<cplx<T>>(), ldc, dout.as<double>(), n2, ctx->stream);
   FC_HIP(hipMemcpyAsync(real_out, dout.as<void>(), sizeof(double) * nrows * n2,
                         hipMemcpyDeviceToHost, ctx->stream));
   FC_HIP(hipStreamSynchronize(ctx->stream));

@@ -224,45 +224,72 @@ __device__ __forceinline__ void fft_lds(cplx<T>* lds, const cplx<T>* __restrict_
 }
 
 // ---- strided column pass ---------------------------------------------------
-// data is [A][L][B] complex with the transform along L:
-//   element (a, e, b) at a*strideA + e*strideL + b.
-// grid = (ceil(B / tc), A); block = tc * (L / E) threads; LDS = L * tc complex.
+// Transforms along L over [A][L][G][B] complex data (G groups of B contiguous
+// columns):
+//   input  element (a, e, g, b) at a*inA  + e*inL  + g*inG  + b,
+//   output element (a, o, g, b) at a*outA + o*outL + g*outG + b,
+// with o in [0, nout).  nout == L writes every output (o = e); nout != L
+// writes only outputs e = (o - off) mod L, o < nout (the pruned inverse: only
+// the lags a separation subgrid needs).  In place when in == out.
+// grid = (G * tilesB, A), tilesB = ceil(B / tc); block = tc * (L / E);
+// LDS = L * tc complex.
 template <typename T, int L, int DIR>
-__global__ void __launch_bounds__(1024) k_fft_cols(cplx<T>* __restrict__ data, int64_t strideA,
-                                                   int64_t strideL, int B, int tc,
-                                                   const cplx<T>* __restrict__ tw) {
+__global__ void __launch_bounds__(1024)
+    k_fft_cols(const cplx<T>* in, int64_t inA, int64_t inL, int64_t inG, cplx<T>* out,
+               int64_t outA, int64_t outL, int64_t outG, int B, int tilesB, int tc, int nout,
+               int off, const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
-  const int b0 = blockIdx.x * tc;
-  C* base = data + (int64_t)blockIdx.y * strideA + b0;
+  const int g = blockIdx.x / tilesB;
+  const int b0 = (blockIdx.x - g * tilesB) * tc;
+  const int a = blockIdx.y;
+  const C* src = in + (int64_t)a * inA + (int64_t)g * inG + b0;
+  C* dst = out + (int64_t)a * outA + (int64_t)g * outG + b0;
   const int nthr = blockDim.x;
   const int tid = threadIdx.x;
   const int ncol = min(tc, B - b0);
-  for (int idx = tid; idx < L * tc; idx += nthr) {
-    const int e = idx / tc, c = idx - e * tc;
-    lds[idx] = c < ncol ? base[e * strideL + c] : C{T(0), T(0)};
+  // L * tc elements over tc * (L / E) threads: exactly E per thread, all
+  // loads issued before the first LDS store.
+  constexpr int E = FftCfg<L>::E;
+  {
+    C tmp[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int idx = tid + q * nthr;
+      const int e = idx / tc, c = idx - e * tc;
+      tmp[q] = c < ncol ? src[e * inL + c] : C{T(0), T(0)};
+    }
+#pragma unroll
+    for (int q = 0; q < E; ++q) lds[tid + q * nthr] = tmp[q];
   }
   __syncthreads();
   fft_lds<T, L, DIR>(lds, tw, tid % tc, tid / tc, LayColFast{tc});
-  for (int idx = tid; idx < L * tc; idx += nthr) {
-    const int e = idx / tc, c = idx - e * tc;
-    if (c < ncol) base[e * strideL + c] = lds[idx];
+  for (int idx = tid; idx < nout * tc; idx += nthr) {
+    const int o = idx / tc, c = idx - o * tc;
+    const int e = nout == L ? o : ((o - off) % L + L) % L;
+    if (c < ncol) dst[o * outL + c] = lds[e * tc + c];
   }
 }
 
-// Direct DFT fallback for one column per block (any L up to LDS capacity).
+// Direct DFT fallback, one column per block (any L up to LDS capacity); same
+// data layout and pruning as k_fft_cols.  grid = (G * B, A).
 template <typename T, int DIR>
-__global__ void __launch_bounds__(256) k_dft_cols(cplx<T>* __restrict__ data, int64_t strideA,
-                                                  int64_t strideL, int B, int L,
-                                                  const cplx<T>* __restrict__ tw) {
+__global__ void __launch_bounds__(256)
+    k_dft_cols(const cplx<T>* in, int64_t inA, int64_t inL, int64_t inG, cplx<T>* out,
+               int64_t outA, int64_t outL, int64_t outG, int B, int L, int nout, int off,
+               const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
-  C* base = data + (int64_t)blockIdx.y * strideA + blockIdx.x;
-  for (int e = threadIdx.x; e < L; e += blockDim.x) lds[e] = base[e * strideL];
+  const int g = blockIdx.x / B;
+  const int b = blockIdx.x - g * B;
+  const C* src = in + (int64_t)blockIdx.y * inA + (int64_t)g * inG + b;
+  C* dst = out + (int64_t)blockIdx.y * outA + (int64_t)g * outG + b;
+  for (int e = threadIdx.x; e < L; e += blockDim.x) lds[e] = src[e * inL];
   __syncthreads();
-  for (int k = threadIdx.x; k < L; k += blockDim.x) {
+  for (int o = threadIdx.x; o < nout; o += blockDim.x) {
+    const int k = nout == L ? o : ((o - off) % L + L) % L;
     T sx = 0, sy = 0;
     int idx = 0;
     for (int e = 0; e < L; ++e) {
@@ -274,7 +301,7 @@ __global__ void __launch_bounds__(256) k_dft_cols(cplx<T>* __restrict__ data, in
       idx += k;
       if (idx >= L) idx -= L;
     }
-    base[k * strideL] = C{sx, sy};  // each block owns its column
+    dst[o * outL] = C{sx, sy};  // each block owns its column
   }
 }
 
@@ -288,8 +315,8 @@ struct LoadCopy {
 
 // block = tr * (M / E) threads, handling rows blockIdx.x*tr ... ; LDS tr*(M+1).
 template <typename T, typename Tin, int M, class Loader>
-__global__ void __launch_bounds__(1024) k_fft_rows_r2c(const Tin* __restrict__ in, int64_t ldi,
-                                                       cplx<T>* __restrict__ out, int64_t ldo,
+__global__ void __launch_bounds__(1024) k_fft_rows_r2c(const Tin* in, int64_t ldi,
+                                                       cplx<T>* out, int64_t ldo,
                                                        int64_t nrows, int tr,
                                                        const cplx<T>* __restrict__ twM,
                                                        const cplx<T>* __restrict__ twN,
@@ -300,16 +327,29 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(const Tin* __restrict__ i
   constexpr int RS = M + 1;
   const int64_t row0 = (int64_t)blockIdx.x * tr;
   const int nthr = blockDim.x, tid = threadIdx.x;
-  for (int idx = tid; idx < tr * M; idx += nthr) {
-    const int c = idx / M, m = idx - c * M;
-    const int64_t row = row0 + c;
-    C z{T(0), T(0)};
-    if (row < nrows) {
-      const Tin* p = in + row * ldi + 2 * m;
-      z.x = T(ld((double)p[0], row, 2 * m));
-      z.y = T(ld((double)p[1], row, 2 * m + 1));
+  // tr * M samples over tr * (M / E) threads: exactly E pairs per thread.
+  {
+    constexpr int E = FftCfg<M>::E;
+    C tmp[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int idx = tid + q * nthr;
+      const int c = idx / M, m = idx - c * M;
+      const int64_t row = row0 + c;
+      C z{T(0), T(0)};
+      if (row < nrows) {
+        const Tin* p = in + row * ldi + 2 * m;
+        z.x = T(ld((double)p[0], row, 2 * m));
+        z.y = T(ld((double)p[1], row, 2 * m + 1));
+      }
+      tmp[q] = z;
     }
-    lds[c * RS + m] = z;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int idx = tid + q * nthr;
+      const int c = idx / M, m = idx - c * M;
+      lds[c * RS + m] = tmp[q];
+    }
   }
   __syncthreads();
   constexpr int TCOL = M / FftCfg<M>::E;
@@ -332,25 +372,48 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(const Tin* __restrict__ i
 
 // Inverse: Hermitian half rows -> real rows (unnormalised; imaginary parts of
 // the DC and Nyquist terms are ignored, as FFTW's c2r does).
+// nsel == 0 writes the whole real row; nsel > 0 writes only samples
+// n = (o - off) mod 2M for o < nsel (the lags a separation subgrid needs).
 template <typename T, typename Tout, int M>
-__global__ void __launch_bounds__(1024) k_fft_rows_c2r(const cplx<T>* __restrict__ in, int64_t ldi,
-                                                       Tout* __restrict__ out, int64_t ldo,
+__global__ void __launch_bounds__(1024) k_fft_rows_c2r(const cplx<T>* in, int64_t ldi,
+                                                       Tout* out, int64_t ldo,
                                                        int64_t nrows, int tr,
                                                        const cplx<T>* __restrict__ twM,
-                                                       const cplx<T>* __restrict__ twN) {
+                                                       const cplx<T>* __restrict__ twN,
+                                                       int nsel, int off) {
   using C = cplx<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   constexpr int RS = M + 1;
   const int64_t row0 = (int64_t)blockIdx.x * tr;
   const int nthr = blockDim.x, tid = threadIdx.x;
-  for (int idx = tid; idx < tr * (M + 1); idx += nthr) {
-    const int c = idx / (M + 1), k = idx - c * (M + 1);
-    const int64_t row = row0 + c;
-    C x{T(0), T(0)};
-    if (row < nrows) x = in[row * ldi + k];
-    if (k == 0 || k == M) x.y = T(0);
-    lds[c * RS + k] = x;
+  {
+    // Elements 0..M-1 of each row: exactly E per thread; element M below.
+    constexpr int E = FftCfg<M>::E;
+    C tmp[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int idx = tid + q * nthr;
+      const int c = idx / M, k = idx - c * M;
+      const int64_t row = row0 + c;
+      C x{T(0), T(0)};
+      if (row < nrows) x = in[row * ldi + k];
+      if (k == 0) x.y = T(0);
+      tmp[q] = x;
+    }
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int idx = tid + q * nthr;
+      const int c = idx / M, k = idx - c * M;
+      lds[c * RS + k] = tmp[q];
+    }
+    if (tid < tr) {
+      const int64_t row = row0 + tid;
+      C x{T(0), T(0)};
+      if (row < nrows) x = in[row * ldi + M];
+      x.y = T(0);
+      lds[tid * RS + M] = x;
+    }
   }
   __syncthreads();
   // Z[k] = (X[k] + conj X[M-k]) + i W^-k (X[k] - conj X[M-k]) for k < M,
@@ -379,21 +442,32 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(const cplx<T>* __restrict
   __syncthreads();
   constexpr int TCOL = M / FftCfg<M>::E;
   fft_lds<T, M, +1>(lds, twM, tid / TCOL, tid % TCOL, LayRow{RS});
-  for (int idx = tid; idx < tr * M; idx += nthr) {
-    const int c = idx / M, m = idx - c * M;
-    const int64_t row = row0 + c;
-    if (row >= nrows) continue;
-    const C z = lds[c * RS + m];
-    Tout* p = out + row * ldo + 2 * m;
-    p[0] = Tout(z.x);
-    p[1] = Tout(z.y);
+  if (nsel == 0) {
+    for (int idx = tid; idx < tr * M; idx += nthr) {
+      const int c = idx / M, m = idx - c * M;
+      const int64_t row = row0 + c;
+      if (row >= nrows) continue;
+      const C z = lds[c * RS + m];
+      Tout* p = out + row * ldo + 2 * m;
+      p[0] = Tout(z.x);
+      p[1] = Tout(z.y);
+    }
+  } else {
+    const T* ldr = reinterpret_cast<const T*>(lds);
+    for (int idx = tid; idx < tr * nsel; idx += nthr) {
+      const int c = idx / nsel, o = idx - c * nsel;
+      const int64_t row = row0 + c;
+      if (row >= nrows) continue;
+      const int n = ((o - off) % (2 * M) + 2 * M) % (2 * M);
+      out[row * ldo + o] = Tout(ldr[2 * (c * RS) + n]);
+    }
   }
 }
 
 // Direct-DFT fallback rows (any n2): one block per row.
 template <typename T, typename Tin, class Loader>
-__global__ void __launch_bounds__(256) k_dft_rows_r2c(const Tin* __restrict__ in, int64_t ldi,
-                                                      cplx<T>* __restrict__ out, int64_t ldo,
+__global__ void __launch_bounds__(256) k_dft_rows_r2c(const Tin* in, int64_t ldi,
+                                                      cplx<T>* out, int64_t ldo,
                                                       int n2, const cplx<T>* __restrict__ tw,
                                                       Loader ld) {
   using C = cplx<T>;
@@ -421,9 +495,10 @@ __global__ void __launch_bounds__(256) k_dft_rows_r2c(const Tin* __restrict__ in
 }
 
 template <typename T, typename Tout>
-__global__ void __launch_bounds__(256) k_dft_rows_c2r(const cplx<T>* __restrict__ in, int64_t ldi,
-                                                      Tout* __restrict__ out, int64_t ldo, int n2,
-                                                      const cplx<T>* __restrict__ tw) {
+__global__ void __launch_bounds__(256) k_dft_rows_c2r(const cplx<T>* in, int64_t ldi,
+                                                      Tout* out, int64_t ldo, int n2,
+                                                      const cplx<T>* __restrict__ tw, int nsel,
+                                                      int off) {
   using C = cplx<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* X = reinterpret_cast<C*>(smem_raw);
@@ -435,7 +510,9 @@ __global__ void __launch_bounds__(256) k_dft_rows_c2r(const cplx<T>* __restrict_
     X[k] = v;
   }
   __syncthreads();
-  for (int n = threadIdx.x; n < n2; n += blockDim.x) {
+  const int nout = nsel == 0 ? n2 : nsel;
+  for (int o = threadIdx.x; o < nout; o += blockDim.x) {
+    const int n = nsel == 0 ? o : ((o - off) % n2 + n2) % n2;
     T s = 0;
     int idx = 0;
     for (int k = 0; k < n2; ++k) {
@@ -445,7 +522,7 @@ __global__ void __launch_bounds__(256) k_dft_rows_c2r(const cplx<T>* __restrict_
       idx += n;
       if (idx >= n2) idx -= n2;
     }
-    out[row * ldo + n] = Tout(s);
+    out[row * ldo + o] = Tout(s);
   }
 }
 

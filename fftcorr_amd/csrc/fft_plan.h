@@ -22,6 +22,9 @@ struct FftPlan {
   int device = 0;
   int n[3] = {0, 0, 0};
   int n2h = 0;
+  // Complex row stride of the work grid: n2/2+1 rounded up to 128 bytes so
+  // every row (and every tile of TC columns) starts on a 128-byte boundary.
+  int ldc = 0;
   int prec = FC_FP64;
   bool fast[3] = {false, false, false};  // axis 2: the half-length row FFT
   int tile[3] = {0, 0, 0};
@@ -30,7 +33,7 @@ struct FftPlan {
   DeviceBuffer tw2M, tw2N;    // lengths n2/2 (fast rows) and n2 (post-process / DFT)
   size_t csize() const { return prec == FC_FP64 ? sizeof(double2) : sizeof(float2); }
   int64_t nrows() const { return (int64_t)n[0] * n[1]; }
-  size_t complex_bytes() const { return (size_t)n[0] * n[1] * n2h * csize(); }
+  size_t complex_bytes() const { return (size_t)n[0] * n[1] * ldc * csize(); }
 };
 
 bool fft_len_fast(int L);
@@ -40,17 +43,30 @@ int fft_len_e(int L);
 std::shared_ptr<FftPlan> get_fft_plan(int device, const int shape[3], int prec);
 
 // ---- launchers -------------------------------------------------------------
+// Column view [A][L][G][B] for k_fft_cols / k_dft_cols (see fft_kernels.h).
+template <typename T>
+struct ColView {
+  const cplx<T>* in;
+  int64_t inA, inL, inG;
+  cplx<T>* out;
+  int64_t outA, outL, outG;
+  int A, G, B;
+  int nout = 0, off = 0;  // nout == 0: all L outputs
+};
+
 template <typename T, int DIR>
-void launch_cols(int L, cplx<T>* data, int64_t strideA, int A, int64_t strideL, int B, int tc,
-                 const cplx<T>* tw, hipStream_t s) {
+void launch_cols(int L, const ColView<T>& v, int tc, const cplx<T>* tw, hipStream_t s) {
   const size_t cs = sizeof(cplx<T>);
-#define FC_COLS_CASE(Lv)                                                                 \
-  case Lv: {                                                                             \
-    const int thr = tc * (Lv / FftCfg<Lv>::E);                                           \
-    dim3 g((B + tc - 1) / tc, A);                                                        \
-    hipLaunchKernelGGL((k_fft_cols<T, Lv, DIR>), g, dim3(thr), (size_t)Lv * tc * cs, s,  \
-                       data, strideA, strideL, B, tc, tw);                               \
-    break;                                                                               \
+  const int nout = v.nout ? v.nout : L;
+#define FC_COLS_CASE(Lv)                                                                  \
+  case Lv: {                                                                              \
+    const int thr = tc * (Lv / FftCfg<Lv>::E);                                            \
+    const int tilesB = (v.B + tc - 1) / tc;                                               \
+    dim3 g((unsigned)(v.G * tilesB), v.A);                                                \
+    hipLaunchKernelGGL((k_fft_cols<T, Lv, DIR>), g, dim3(thr), (size_t)Lv * tc * cs, s,   \
+                       v.in, v.inA, v.inL, v.inG, v.out, v.outA, v.outL, v.outG, v.B,     \
+                       tilesB, tc, nout, v.off, tw);                                      \
+    break;                                                                                \
   }
   switch (L) {
     FC_COLS_CASE(8)
@@ -72,13 +88,28 @@ void launch_cols(int L, cplx<T>* data, int64_t strideA, int A, int64_t strideL, 
     FC_COLS_CASE(1536)
     FC_COLS_CASE(3072)
     default: {
-      dim3 g(B, A);
-      hipLaunchKernelGGL((k_dft_cols<T, DIR>), g, dim3(256), (size_t)L * cs, s, data, strideA,
-                         strideL, B, L, tw);
+      dim3 g((unsigned)(v.G * v.B), v.A);
+      hipLaunchKernelGGL((k_dft_cols<T, DIR>), g, dim3(256), (size_t)L * cs, s, v.in, v.inA,
+                         v.inL, v.inG, v.out, v.outA, v.outL, v.outG, v.B, L, nout, v.off, tw);
     }
   }
 #undef FC_COLS_CASE
   FC_LAUNCH_CHECK();
+}
+
+// In-place y pass over the work grid [n0][n1][ldc].
+template <typename T>
+ColView<T> y_view(const FftPlan& p, cplx<T>* d) {
+  const int64_t ldc = p.ldc;
+  return ColView<T>{d, (int64_t)p.n[1] * ldc, ldc, 0, d, (int64_t)p.n[1] * ldc, ldc, 0,
+                    p.n[0], 1, p.n2h};
+}
+// In-place x pass over the work grid: L = n0, groups = the n1 rows of each slab.
+template <typename T>
+ColView<T> x_view(const FftPlan& p, cplx<T>* d) {
+  const int64_t ldc = p.ldc;
+  return ColView<T>{d, 0, (int64_t)p.n[1] * ldc, ldc, d, 0, (int64_t)p.n[1] * ldc, ldc,
+                    1, p.n[1], p.n2h};
 }
 
 template <typename T, typename Tin, class Loader>
@@ -129,11 +160,13 @@ void launch_rows_r2c(const FftPlan& p, const Tin* in, int64_t ldi, cplx<T>* out,
   FC_LAUNCH_CHECK();
 }
 
+// nrows rows of ldi complex -> real rows of ldo; nsel > 0 keeps only samples
+// (o - off) mod n2, o < nsel (pruned inverse).
 template <typename T, typename Tout>
 void launch_rows_c2r(const FftPlan& p, const cplx<T>* in, int64_t ldi, Tout* out, int64_t ldo,
-                     hipStream_t s) {
+                     hipStream_t s, int64_t nrows = -1, int nsel = 0, int off = 0) {
   const size_t cs = sizeof(cplx<T>);
-  const int64_t nrows = p.nrows();
+  if (nrows < 0) nrows = p.nrows();
   const int M = p.n[2] / 2;
   const int tr = p.tile[2];
   const cplx<T>* twM = p.tw2M.as<cplx<T>>();
@@ -145,7 +178,7 @@ void launch_rows_c2r(const FftPlan& p, const cplx<T>* in, int64_t ldi, Tout* out
     dim3 g((unsigned)((nrows + tr - 1) / tr));                                             \
     hipLaunchKernelGGL((k_fft_rows_c2r<T, Tout, Mv>), g, dim3(thr),                        \
                        (size_t)tr * (Mv + 1) * cs, s, in, ldi, out, ldo, nrows, tr, twM,    \
-                       twN);                                                               \
+                       twN, nsel, off);                                                    \
     break;                                                                                 \
   }
     switch (M) {
@@ -173,29 +206,23 @@ void launch_rows_c2r(const FftPlan& p, const cplx<T>* in, int64_t ldi, Tout* out
     dim3 g((unsigned)nrows);
     const int nh = p.n[2] / 2 + 1;
     hipLaunchKernelGGL((k_dft_rows_c2r<T, Tout>), g, dim3(256), (size_t)nh * cs, s, in, ldi,
-                       out, ldo, p.n[2], twN);
+                       out, ldo, p.n[2], twN, nsel, off);
   }
   FC_LAUNCH_CHECK();
 }
 
-// y then x forward passes over the complex work grid [n0][n1][n2h].
+// y then x forward passes over the complex work grid [n0][n1][ldc].
 template <typename T>
 void fft_yx_forward(const FftPlan& p, cplx<T>* data, hipStream_t s) {
-  const int64_t n2h = p.n2h;
-  launch_cols<T, -1>(p.n[1], data, (int64_t)p.n[1] * n2h, p.n[0], n2h, (int)n2h, p.tile[1],
-                     p.tw1.as<cplx<T>>(), s);
-  launch_cols<T, -1>(p.n[0], data, 0, 1, (int64_t)p.n[1] * n2h, (int)(p.n[1] * n2h),
-                     p.tile[0], p.tw0.as<cplx<T>>(), s);
+  launch_cols<T, -1>(p.n[1], y_view<T>(p, data), p.tile[1], p.tw1.as<cplx<T>>(), s);
+  launch_cols<T, -1>(p.n[0], x_view<T>(p, data), p.tile[0], p.tw0.as<cplx<T>>(), s);
 }
 
 // x then y inverse passes.
 template <typename T>
 void fft_xy_inverse(const FftPlan& p, cplx<T>* data, hipStream_t s) {
-  const int64_t n2h = p.n2h;
-  launch_cols<T, +1>(p.n[0], data, 0, 1, (int64_t)p.n[1] * n2h, (int)(p.n[1] * n2h),
-                     p.tile[0], p.tw0.as<cplx<T>>(), s);
-  launch_cols<T, +1>(p.n[1], data, (int64_t)p.n[1] * n2h, p.n[0], n2h, (int)n2h, p.tile[1],
-                     p.tw1.as<cplx<T>>(), s);
+  launch_cols<T, +1>(p.n[0], x_view<T>(p, data), p.tile[0], p.tw0.as<cplx<T>>(), s);
+  launch_cols<T, +1>(p.n[1], y_view<T>(p, data), p.tile[1], p.tw1.as<cplx<T>>(), s);
 }
 
 }  // namespace fc

@@ -114,6 +114,92 @@ __global__ void k_hist_final(const double* __restrict__ part_hist,
   }
 }
 
+// Cells of an [s0, s1, s2] subgrid grouped by histogram bin.  The bin of a
+// cell depends only on the subgrid axes (correlator.h:129-135, :180-186 and
+// histogram_list.h:57), so it is computed once on the host, in FP64 exactly
+// as the reference does, and the cells are stored sorted by bin (raster
+// order within a bin).  A wave then reads 64 consecutive entries -- almost
+// always one bin -- and the per-bin counts are known without counting.
+struct BinnedCells {
+  DeviceBuffer cells;        // uint32 linear index (i*s1 + j)*s2 + k, sorted by bin
+  DeviceBuffer start;        // int32 [nbins + 1]: first entry of each bin
+  DeviceBuffer counts;       // int32 [nbins]
+  std::vector<int> h_counts;
+  int64_t n = 0;             // entries (cells whose bin is in range)
+  int nbins = 0;
+};
+
+void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
+                        const std::vector<double>& ay, const std::vector<double>& az,
+                        double binsize, int nbins);
+
+__device__ __forceinline__ int bin_of_entry(const int* __restrict__ start, int nbins,
+                                            int64_t pos) {
+  int lo = 0, hi = nbins - 1;  // largest b with start[b] <= pos
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((int64_t)start[mid] <= pos)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Histogram over bin-sorted cells: F(cell, v) fills NV values for a cell.
+template <int NV, class F>
+__global__ void __launch_bounds__(256)
+    k_hist_sorted(F f, const uint32_t* __restrict__ cells, const int* __restrict__ start,
+                  int64_t n, int nbins, double* __restrict__ part_hist) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int nw = blockDim.x / kWave;
+  double* bins = reinterpret_cast<double*>(smem_raw);
+  int* lcounts = reinterpret_cast<int*>(bins + (size_t)nw * NV * nbins);
+  for (int i = threadIdx.x; i < nw * NV * nbins; i += blockDim.x) bins[i] = 0.0;
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) lcounts[i] = 0;
+  __syncthreads();
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < n ? lo + chunk : n;
+  double* wb = bins + (size_t)(threadIdx.x / kWave) * NV * nbins;
+  const int64_t rounds = hi > lo ? (hi - lo + blockDim.x - 1) / blockDim.x : 0;
+  for (int64_t r = 0; r < rounds; ++r) {
+    const int64_t pos = lo + r * blockDim.x + threadIdx.x;
+    int bin = -1;
+    double v[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = 0.0;
+    if (pos < hi) {
+      bin = bin_of_entry(start, nbins, pos);
+      f((int64_t)cells[pos], v);
+    }
+    wave_hist_add<NV>(bin, v, wb, lcounts, nbins);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NV * nbins; i += blockDim.x) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += bins[(size_t)w * NV * nbins + i];
+    part_hist[(size_t)blockIdx.x * NV * nbins + i] = s;
+  }
+}
+
+// hist rows += block partials (in block order); count rows += the static counts.
+template <int NV>
+__global__ void k_hist_final_static(const double* __restrict__ part_hist, int nblocks, int nbins,
+                                    const int* __restrict__ counts_in,
+                                    double* __restrict__ hist, int* __restrict__ counts,
+                                    int row0, int crow0, int crow1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < NV * nbins) {
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part_hist[(size_t)b * NV * nbins + i];
+    const int q = i / nbins, bin = i % nbins;
+    hist[(size_t)(row0 + q) * nbins + bin] += s;
+  }
+  if (i < nbins)
+    for (int r = crow0; r < crow1; ++r) counts[(size_t)r * nbins + i] += counts_in[i];
+}
+
 // Scratch for one histogram pass.
 struct HistScratch {
   DeviceBuffer part_hist, part_counts;
@@ -122,6 +208,26 @@ struct HistScratch {
     part_counts.reserve(sizeof(int) * (size_t)kHistBlocks * nbins);
   }
 };
+
+template <int NV, class F>
+void run_hist_sorted(F f, const BinnedCells& bc, HistScratch& hs, double* hist, int* counts,
+                     int row0, int crow0, int crow1, hipStream_t s) {
+  const int nbins = bc.nbins;
+  const int thr = hist_threads(NV, nbins);
+  if (thr == 0) fail(FC_ERR_INVALID, "too many histogram bins (" + std::to_string(nbins) + ")");
+  hs.reserve(NV, nbins);
+  int nb = (int)std::min<int64_t>(kHistBlocks, (bc.n + thr - 1) / thr);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL((k_hist_sorted<NV, F>), dim3(nb), dim3(thr), hist_lds_bytes(NV, nbins, thr),
+                     s, f, bc.cells.as<uint32_t>(), bc.start.as<int>(), bc.n, nbins,
+                     hs.part_hist.as<double>());
+  FC_LAUNCH_CHECK();
+  const int tot = NV * nbins > nbins ? NV * nbins : nbins;
+  hipLaunchKernelGGL((k_hist_final_static<NV>), dim3((tot + 255) / 256), dim3(256), 0, s,
+                     hs.part_hist.as<double>(), nb, nbins, bc.counts.as<int>(), hist, counts,
+                     row0, crow0, crow1);
+  FC_LAUNCH_CHECK();
+}
 
 template <int NV, class F>
 void run_hist(F f, int64_t n, int nbins, HistScratch& hs, double* hist, int* counts, int row0,
