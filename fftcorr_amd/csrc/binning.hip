@@ -8,7 +8,7 @@ namespace fc {
 
 void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
                         const std::vector<double>& ay, const std::vector<double>& az,
-                        double binsize, int nbins) {
+                        double binsize, int nbins, int64_t track_cell) {
   const int s0 = (int)ax.size(), s1 = (int)ay.size(), s2 = (int)az.size();
   if ((uint64_t)s0 * s1 * s2 > 0xffffffffULL) fail(FC_ERR_INVALID, "subgrid too large");
   // Bin of cell (i, j, k): floor((|r| - 0) / binsize) with
@@ -59,6 +59,7 @@ void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
   start[nbins] = pos;
   if (pos > 0x7fffffffLL) fail(FC_ERR_INVALID, "subgrid too large");
   std::vector<uint32_t> cells(pos > 0 ? pos : 1);
+  std::vector<int> binv(pos > 0 ? pos : 1);
   {
     std::vector<std::thread> th;
     for (unsigned t = 0; t < nt; ++t)
@@ -70,21 +71,29 @@ void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
           for (int j = 0; j < s1; ++j)
             for (int k = 0; k < s2; ++k) {
               const int b = bin_of(i, j, k);
-              if (b >= 0) cells[o[b]++] = (uint32_t)(((int64_t)i * s1 + j) * s2 + k);
+              if (b >= 0) {
+                binv[o[b]] = b;
+                cells[o[b]++] = (uint32_t)(((int64_t)i * s1 + j) * s2 + k);
+              }
             }
       });
     for (auto& x : th) x.join();
   }
-  std::vector<int> start32(nbins + 1);
-  for (int b = 0; b <= nbins; ++b) start32[b] = (int)start[b];
+  bc.tracked_pos = -1;
+  if (track_cell >= 0)
+    for (int64_t q = 0; q < pos; ++q)
+      if ((int64_t)cells[q] == track_cell) {
+        bc.tracked_pos = q;
+        break;
+      }
   bc.n = pos;
   bc.nbins = nbins;
   bc.cells.allocate(sizeof(uint32_t) * cells.size());
-  bc.start.allocate(sizeof(int) * (nbins + 1));
+  bc.bins.allocate(sizeof(int) * binv.size());
   bc.counts.allocate(sizeof(int) * nbins);
   FC_HIP(hipMemcpy(bc.cells.as<void>(), cells.data(), sizeof(uint32_t) * cells.size(),
                    hipMemcpyHostToDevice));
-  FC_HIP(hipMemcpy(bc.start.as<void>(), start32.data(), sizeof(int) * (nbins + 1),
+  FC_HIP(hipMemcpy(bc.bins.as<void>(), binv.data(), sizeof(int) * binv.size(),
                    hipMemcpyHostToDevice));
   FC_HIP(hipMemcpy(bc.counts.as<void>(), bc.h_counts.data(), sizeof(int) * nbins,
                    hipMemcpyHostToDevice));

@@ -12,9 +12,10 @@
 //   pruned inverse y pass, pruned z C2R straight into the r subgrid;
 //   r-space multipoles (extract_submatrix + Y_l0(r) + histogram).
 // Correlator (Yamamoto, observer at the survey origin), per (l, m):
-//   z pass with Y_lm(x) * delta fused into the load -> y, x -> conj-multiply
-//   with the stored FFT of grid 2 -> accumulate the k subgrid -> inverse ->
-//   accumulate the r subgrid; histograms once per l.
+//   z pass with Y_lm(x) * delta fused into the load -> y -> the same fused x
+//   pass with F conj(G2) against the stored FFT of grid 2 -> pruned inverse
+//   y, z -> accumulate the k and r subgrids with Y_lm(k), Y_lm(r);
+//   histograms once per l.
 // Work arrays are FP64 (FC_FP64) or FP32 (FC_FP32); Y_lm, windows, bin
 // indices and every sum stay in FP64.
 #include <algorithm>
@@ -44,6 +45,7 @@ struct fc_corr {
   std::vector<double> h_rx, h_kx, h_ky, h_kz;
   fc::DeviceBuffer rx, kx, ky, kz, wx, wy, wz;  // FP64 axes and per-axis window W^2
   fc::BinnedCells kbin, rbin;                   // subgrid cells sorted by bin
+  fc::DeviceBuffer ksrc, kw, rw;                // periodic: pk index + weights per entry
   // FFT plan + work arrays (complex rows of plan->ldc).
   std::shared_ptr<fc::FftPlan> plan;
   fc::DeviceBuffer work, grid2_fft, work_b;
@@ -272,118 +274,83 @@ struct LoadYlm {
   }
 };
 
-// ---- full-grid subgrid index maps (Yamamoto; cc/grid/fft_grid.cc:182-257) --
-struct KMap {
-  int n0, n1;
-  int64_t ldc;
-  int ks0, ks1, ks2;
-  // Half-complex element (a, b, c) feeding k-subgrid cell (i, j, k).
-  __device__ __forceinline__ int64_t src(int i, int j, int k) const {
-    const int ox = ks0 / 2, oy = ks1 / 2, oz = ks2 / 2;
-    const int ii = ((n0 - ox + i) % n0 + n0) % n0;
-    const int jj = ((n1 - oy + j) % n1 + n1) % n1;
-    if (k < oz) {  // negative kz: reflected (-kx, -ky, |kz|) row
-      const int iin = (n0 - ii) % n0, jjn = (n1 - jj) % n1;
-      return ((int64_t)iin * n1 + jjn) * ldc + (oz - k);
-    }
-    return ((int64_t)ii * n1 + jj) * ldc + (k - oz);
-  }
-};
-
-struct RMap {
-  int n0, n1, n2;
-  int64_t ld;  // real row stride of the padded work grid
-  int rs;
-  __device__ __forceinline__ int64_t src(int i, int j, int k) const {
-    // Periodic image of lag (i - o, j - o, k - o); the positive modulo also
-    // covers subgrids wider than the grid (rmax beyond half the box).
-    const int o = rs / 2;
-    const int ii = ((n0 - o + i) % n0 + n0) % n0;
-    const int jj = ((n1 - o + j) % n1 + n1) % n1;
-    const int kk = ((n2 - o + k) % n2 + n2) % n2;
-    return ((int64_t)ii * n1 + jj) * ld + kk;
-  }
-};
-
-// Periodic k-space multipoles, every l at once (correlator.h:296-320), from
-// the compact array.
-template <typename T, int NV>
-struct PeriodicKVals {
-  const T* pk;
-  KSlots ks;
-  const double *kx, *ky, *kz, *wx, *wy, *wz;
-  double c[NV];
-  __device__ __forceinline__ void operator()(int64_t cell, double* v) const {
+// Static weights of the periodic multipole histograms, one per sorted
+// entry and multipole: the k-space weight Y_l0(k) W^-1(k) sqrt(4 pi (2l+1))
+// cell^3 / N (correlator.h:296-320) and the pk index the entry reads; the
+// r-space weight Y_l0(r) sqrt(4 pi (2l+1)) / N^2 (correlator.h:336-355).  A
+// histogram sample is then weight * value, the same product, in the same
+// FP64 operation order, as the reference's make_ylm followed by extraction.
+template <int NV>
+__global__ void k_setup_kweights(const uint32_t* __restrict__ cells, int64_t n, KSlots ks,
+                                 const double* __restrict__ kx, const double* __restrict__ ky,
+                                 const double* __restrict__ kz, const double* __restrict__ wx,
+                                 const double* __restrict__ wy, const double* __restrict__ wz,
+                                 const double* __restrict__ coef, uint32_t* __restrict__ src,
+                                 double* __restrict__ wt) {
 #pragma clang fp contract(off)
+  const int oz = ks.oz();
+  for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < n;
+       pos += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t cell = cells[pos];
     const int k = (int)(cell % ks.ks2);
     const int64_t t = cell / ks.ks2;
     const int j = (int)(t % ks.ks1), i = (int)(t / ks.ks1);
-    const double x = kx[i], y = ky[j], z = kz[k];
-    const int oz = ks.oz();
-    const int64_t src = k >= oz ? ks.at(i, j, k - oz)
-                                : ks.at(ks.ks0 - 1 - i, ks.ks1 - 1 - j, oz - k);
-    const double re = (double)pk[src];
+    src[pos] = (uint32_t)(k >= oz ? ks.at(i, j, k - oz)
+                                  : ks.at(ks.ks0 - 1 - i, ks.ks1 - 1 - j, oz - k));
     const double invw = 1.0 / (wx[i] * wy[j] * wz[k]);
 #pragma unroll
-    for (int q = 0; q < NV; ++q) v[q] = ylm_eval(2 * q, 0, x, y, z, c[q], invw) * re;
+    for (int q = 0; q < NV; ++q)
+      wt[(int64_t)q * n + pos] = ylm_eval(2 * q, 0, kx[i], ky[j], kz[k], coef[q], invw);
   }
-};
+}
 
-// Periodic r-space multipoles, every l at once (correlator.h:336-355), from
-// the r subgrid xi[rs][rs][rs].
-template <typename T, int NV>
-struct PeriodicRVals {
-  const T* xi;
-  int rs;
-  const double* rx;
-  double c[NV];
-  double* zerolag;
-  __device__ __forceinline__ void operator()(int64_t cell, double* v) const {
+template <int NV>
+__global__ void k_setup_rweights(const uint32_t* __restrict__ cells, int64_t n, int rs,
+                                 const double* __restrict__ rx, const double* __restrict__ coef,
+                                 double* __restrict__ wt) {
 #pragma clang fp contract(off)
+  for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < n;
+       pos += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t cell = cells[pos];
     const int k = (int)(cell % rs);
     const int64_t t = cell / rs;
     const int j = (int)(t % rs), i = (int)(t / rs);
-    const double x = rx[i], y = rx[j], z = rx[k];
-    const double val = (double)xi[cell];
-    const int o = rs / 2;
-    if (i == o && j == o && k == o) *zerolag = ylm_eval(0, 0, x, y, z, c[0], 1.0) * val;
 #pragma unroll
-    for (int q = 0; q < NV; ++q) v[q] = ylm_eval(2 * q, 0, x, y, z, c[q], 1.0) * val;
+    for (int q = 0; q < NV; ++q)
+      wt[(int64_t)q * n + pos] = ylm_eval(2 * q, 0, rx[i], rx[j], rx[k], coef[q], 1.0);
   }
-};
+}
 
-// Histogram of an FP64 subgrid (one multipole row).
-struct SubgridVals {
-  const double* sub;
-  __device__ __forceinline__ void operator()(int64_t cell, double* v) const { v[0] = sub[cell]; }
-};
-
-// Yamamoto accumulation into the k subgrid (correlator.h:453-466).
+// Yamamoto accumulation into the k subgrid (correlator.h:453-466) from the
+// compact array: ksub += Y_lm(k) 4 pi W^-1(k) Re P.
 template <typename T>
-__global__ void k_acc_k(double* __restrict__ ksub, const cplx<T>* __restrict__ P, KMap map,
+__global__ void k_acc_k(double* __restrict__ ksub, const T* __restrict__ pk, KSlots ks,
                         const double* __restrict__ kx, const double* __restrict__ ky,
                         const double* __restrict__ kz, const double* __restrict__ wx,
                         const double* __restrict__ wy, const double* __restrict__ wz, int ell,
                         int m, double c) {
 #pragma clang fp contract(off)
-  const int64_t n = (int64_t)map.ks0 * map.ks1 * map.ks2;
+  const int64_t n = (int64_t)ks.ks0 * ks.ks1 * ks.ks2;
+  const int oz = ks.oz();
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    const int k = (int)(idx % map.ks2);
-    const int64_t t = idx / map.ks2;
-    const int j = (int)(t % map.ks1), i = (int)(t / map.ks1);
+    const int k = (int)(idx % ks.ks2);
+    const int64_t t = idx / ks.ks2;
+    const int j = (int)(t % ks.ks1), i = (int)(t / ks.ks1);
+    const int64_t src = k >= oz ? ks.at(i, j, k - oz)
+                                : ks.at(ks.ks0 - 1 - i, ks.ks1 - 1 - j, oz - k);
     const double invw = 1.0 / (wx[i] * wy[j] * wz[k]);
     const double y = ylm_eval(ell, m, kx[i], ky[j], kz[k], c, invw);
-    ksub[idx] += y * (double)P[map.src(i, j, k)].x;
+    ksub[idx] += y * (double)pk[src];
   }
 }
 
-// Yamamoto accumulation into the r subgrid (correlator.h:477-483).
+// Yamamoto accumulation into the r subgrid (correlator.h:477-483) from the
+// pruned inverse: rsub += Y_lm(r) 4 pi / N xi.
 template <typename T>
-__global__ void k_acc_r(double* __restrict__ rsub, const T* __restrict__ xi, RMap map,
+__global__ void k_acc_r(double* __restrict__ rsub, const T* __restrict__ xi, int rs,
                         const double* __restrict__ rx, int ell, int m, double c) {
 #pragma clang fp contract(off)
-  const int rs = map.rs;
   const int64_t n = (int64_t)rs * rs * rs;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
        idx += (int64_t)gridDim.x * blockDim.x) {
@@ -391,7 +358,7 @@ __global__ void k_acc_r(double* __restrict__ rsub, const T* __restrict__ xi, RMa
     const int64_t t = idx / rs;
     const int j = (int)(t % rs), i = (int)(t / rs);
     const double y = ylm_eval(ell, m, rx[i], rx[j], rx[k], c, 1.0);
-    rsub[idx] += y * (double)xi[map.src(i, j, k)];
+    rsub[idx] += y * (double)xi[idx];
   }
 }
 
@@ -521,7 +488,9 @@ void setup_geometry(fc_corr* c) {
     for (int k = 0; k < c->kshape[i]; ++k) w[i][k] = window_sq(c->window, (*axes[i])[k], c->cell);
   }
   build_binned_cells(c->kbin, c->h_kx, c->h_ky, c->h_kz, c->dk, c->nkbins);
-  build_binned_cells(c->rbin, c->h_rx, c->h_rx, c->h_rx, c->dr, c->nrbins);
+  const int64_t o = c->rmax_cells;
+  build_binned_cells(c->rbin, c->h_rx, c->h_rx, c->h_rx, c->dr, c->nrbins,
+                     (o * c->rshape + o) * c->rshape + o);
   upload(c->rx, c->h_rx);
   upload(c->kx, c->h_kx);
   upload(c->ky, c->h_ky);
@@ -590,23 +559,6 @@ template <typename T, class Loader>
 void forward_fft(fc_corr* c, fc_grid* g, cplx<T>* dst, Loader ld) {
   forward_zy<T>(c, g, dst, ld);
   forward_x<T>(c, dst);
-}
-
-// Full inverse 3D FFT in place: complex rows -> padded real rows (2 ldc).
-template <typename T>
-void inverse_fft(fc_corr* c, cplx<T>* data) {
-  const FftPlan& p = *c->plan;
-  hipStream_t s = c->ctx->stream;
-  {
-    Stage st(c, FC_STAGE_FFT_X_INV);
-    launch_cols<T, +1>(p.n[0], x_view<T>(p, data), p.tile[0], p.tw0.as<cplx<T>>(), s);
-  }
-  {
-    Stage st(c, FC_STAGE_FFT_Y_INV);
-    launch_cols<T, +1>(p.n[1], y_view<T>(p, data), p.tile[1], p.tw1.as<cplx<T>>(), s);
-  }
-  Stage st(c, FC_STAGE_FFT_Z_INV);
-  launch_rows_c2r<T, T>(p, data, p.ldc, reinterpret_cast<T*>(data), 2 * (int64_t)p.ldc, s);
 }
 
 // exp(i pi f / n) for the signed frequency f of every index along each axis
@@ -684,63 +636,114 @@ void launch_periodic_x(fc_corr* c, cplx<T>* a, const cplx<T>* b) {
   FC_LAUNCH_CHECK();
 }
 
-template <typename T, int NV>
-void periodic_tail(fc_corr* c) {
+// |F|^2 (or F conj(G)) -> compact k values pk, then the pruned inverse
+// (x lags in the fused kernel, then y and z) -> r subgrid xisub.
+template <typename T>
+void power_and_pruned_inverse(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode) {
   hipStream_t s = c->ctx->stream;
   const FftPlan& p = *c->plan;
+  const int64_t nc = p.nrows() * p.ldc;
+  const int64_t ldc = p.ldc;
   const int rs = c->rshape, o = rs / 2;
-  const uint64_t ncells = (uint64_t)c->shape[0] * c->shape[1] * c->shape[2];
-  // correlator.h:294 and :310-313: k_rescale = cell^3 / N.
-  const double k_rescale = c->cell * c->cell * c->cell / ncells;
-  PeriodicKVals<T, NV> kv;
-  kv.pk = c->pk.as<T>();
-  kv.ks = kslots(c);
-  kv.kx = c->kx.as<double>();
-  kv.ky = c->ky.as<double>();
-  kv.kz = c->kz.as<double>();
-  kv.wx = c->wx.as<double>();
-  kv.wy = c->wy.as<double>();
-  kv.wz = c->wz.as<double>();
-  for (int q = 0; q < NV; ++q)
-    kv.c[q] = ylm_coeff(2 * q, 0, std::sqrt((4.0 * M_PI) * (2 * (2 * q) + 1)) * k_rescale);
-  {
-    Stage st(c, FC_STAGE_HIST_K);
-    run_hist_sorted<NV>(kv, c->kbin, c->hs, c->k_hist.as<double>(), c->k_counts.as<int>(), 0, 0,
-                        NV, s);
+  if (p.fast[0]) {
+    Stage st(c, FC_STAGE_COMBINE);
+    if (mode == kModeInterlaced)
+      launch_periodic_x<T, kModeInterlaced>(c, w, b);
+    else if (mode == kModeCross)
+      launch_periodic_x<T, kModeCross>(c, w, b);
+    else
+      launch_periodic_x<T, kModeAuto>(c, w, b);
+  } else {
+    // Unusual x length: separate x pass, combine, gather, pruned inverse x.
+    forward_x<T>(c, w);
+    if (mode == kModeInterlaced) forward_x<T>(c, const_cast<cplx<T>*>(b));
+    Stage st(c, FC_STAGE_COMBINE);
+    if (mode == kModeInterlaced) {
+      hipLaunchKernelGGL(k_interlace<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, w, b,
+                         p.n[0], p.n[1], ldc, c->ph[0].as<double2>(), c->ph[1].as<double2>(),
+                         c->ph[2].as<double2>());
+      FC_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_conj_mult<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, w,
+                       mode == kModeCross ? b : nullptr, nc);
+    FC_LAUNCH_CHECK();
+    const KSlots ks = kslots(c);
+    const int64_t npk = (int64_t)ks.ks0 * ks.ks1 * (ks.ks2 / 2 + 1);
+    hipLaunchKernelGGL(k_gather_pk<T>, dim3(grid1d(npk, 256, 65536)), dim3(256), 0, s,
+                       c->pk.as<T>(), w, ks, p.n[0], p.n[1], ldc);
+    FC_LAUNCH_CHECK();
+    ColView<T> v{w, 0, (int64_t)p.n[1] * ldc, ldc, c->q1.as<cplx<T>>(), 0,
+                 (int64_t)p.n[1] * ldc, ldc, 1, p.n[1], p.n2h, rs, o};
+    launch_cols<T, +1>(p.n[0], v, p.tile[0], p.tw0.as<cplx<T>>(), s);
   }
   // Pruned inverse y pass: q1[sx][y][c] -> q2[sx][sy][c], only the rs lags.
   {
     Stage st(c, FC_STAGE_FFT_Y_INV);
-    const int64_t ldc = p.ldc;
     ColView<T> v{c->q1.as<cplx<T>>(), (int64_t)p.n[1] * ldc, ldc, 0, c->q2.as<cplx<T>>(),
                  (int64_t)rs * ldc, ldc, 0, rs, 1, p.n2h, rs, o};
     launch_cols<T, +1>(p.n[1], v, p.tile[1], p.tw1.as<cplx<T>>(), s);
   }
   // Pruned z C2R: rows of q2 -> r subgrid rows xi[sx][sy][sz].
-  {
-    Stage st(c, FC_STAGE_FFT_Z_INV);
-    launch_rows_c2r<T, T>(p, c->q2.as<cplx<T>>(), p.ldc, c->xisub.as<T>(), rs, s,
-                          (int64_t)rs * rs, rs, o);
-  }
-  // correlator.h:336: r_rescale = 1 / N / N.
+  Stage st(c, FC_STAGE_FFT_Z_INV);
+  launch_rows_c2r<T, T>(p, c->q2.as<cplx<T>>(), p.ldc, c->xisub.as<T>(), rs, s,
+                        (int64_t)rs * rs, rs, o);
+}
+
+// Builds the periodic histogram weights once per correlator.
+template <int NV>
+void ensure_periodic_weights(fc_corr* c) {
+  if (c->kw.bytes()) return;
+  const uint64_t ncells = (uint64_t)c->shape[0] * c->shape[1] * c->shape[2];
+  // correlator.h:294, :310-313 (k_rescale = cell^3 / N) and :336 (1 / N / N).
+  const double k_rescale = c->cell * c->cell * c->cell / ncells;
   const double r_rescale = (1.0 / ncells / ncells);
-  PeriodicRVals<T, NV> rv;
-  rv.xi = c->xisub.as<T>();
-  rv.rs = rs;
-  rv.rx = c->rx.as<double>();
-  for (int q = 0; q < NV; ++q)
-    rv.c[q] = ylm_coeff(2 * q, 0, std::sqrt((4.0 * M_PI) * (2 * (2 * q) + 1)) * r_rescale);
-  rv.zerolag = c->zerolag.as<double>();
+  double kc[NV], rc[NV];
+  for (int q = 0; q < NV; ++q) {
+    kc[q] = ylm_coeff(2 * q, 0, std::sqrt((4.0 * M_PI) * (2 * (2 * q) + 1)) * k_rescale);
+    rc[q] = ylm_coeff(2 * q, 0, std::sqrt((4.0 * M_PI) * (2 * (2 * q) + 1)) * r_rescale);
+  }
+  DeviceBuffer coef(sizeof(double) * 2 * NV);
+  std::vector<double> h(kc, kc + NV);
+  h.insert(h.end(), rc, rc + NV);
+  FC_HIP(hipMemcpy(coef.as<void>(), h.data(), sizeof(double) * 2 * NV, hipMemcpyHostToDevice));
+  const int64_t nk = c->kbin.n, nr = c->rbin.n;
+  c->ksrc.allocate(sizeof(uint32_t) * std::max<int64_t>(nk, 1));
+  c->kw.allocate(sizeof(double) * NV * std::max<int64_t>(nk, 1));
+  c->rw.allocate(sizeof(double) * NV * std::max<int64_t>(nr, 1));
+  hipStream_t s = c->ctx->stream;
+  if (nk > 0)
+    hipLaunchKernelGGL((k_setup_kweights<NV>), dim3(grid1d(nk, 256, 65536)), dim3(256), 0, s,
+                       c->kbin.cells.as<uint32_t>(), nk, kslots(c), c->kx.as<double>(),
+                       c->ky.as<double>(), c->kz.as<double>(), c->wx.as<double>(),
+                       c->wy.as<double>(), c->wz.as<double>(), coef.as<double>(),
+                       c->ksrc.as<uint32_t>(), c->kw.as<double>());
+  FC_LAUNCH_CHECK();
+  if (nr > 0)
+    hipLaunchKernelGGL((k_setup_rweights<NV>), dim3(grid1d(nr, 256, 65536)), dim3(256), 0, s,
+                       c->rbin.cells.as<uint32_t>(), nr, c->rshape, c->rx.as<double>(),
+                       coef.as<double>() + NV, c->rw.as<double>());
+  FC_LAUNCH_CHECK();
+  FC_HIP(hipStreamSynchronize(s));  // coef is freed on return
+}
+
+template <typename T, int NV>
+void periodic_hists(fc_corr* c) {
+  hipStream_t s = c->ctx->stream;
+  ensure_periodic_weights<NV>(c);
+  {
+    Stage st(c, FC_STAGE_HIST_K);
+    run_hist_static<NV, T>(c->pk.as<T>(), c->ksrc.as<uint32_t>(), c->kw.as<double>(), c->kbin,
+                           c->hs, c->k_hist.as<double>(), c->k_counts.as<int>(), 0, 0, NV, s);
+  }
   Stage st(c, FC_STAGE_HIST_R);
-  run_hist_sorted<NV>(rv, c->rbin, c->hs, c->r_hist.as<double>(), c->r_counts.as<int>(), 0, 0,
-                      NV, s);
+  run_hist_static<NV, T>(c->xisub.as<T>(), c->rbin.cells.as<uint32_t>(), c->rw.as<double>(),
+                         c->rbin, c->hs, c->r_hist.as<double>(), c->r_counts.as<int>(), 0, 0, NV,
+                         s, c->rbin.tracked_pos, c->zerolag.as<double>());
 }
 
 template <typename T>
 void periodic_correlate(fc_corr* c, fc_grid* g1, int mode, fc_grid* gb) {
-  hipStream_t s = c->ctx->stream;
   const FftPlan& p = *c->plan;
-  const int64_t nc = p.nrows() * p.ldc;
   cplx<T>* w = c->work.as<cplx<T>>();
   ensure_periodic_buffers<T>(c);
   ensure_phase_tables(c);
@@ -753,47 +756,16 @@ void periodic_correlate(fc_corr* c, fc_grid* g1, int mode, fc_grid* gb) {
   } else if (mode == kModeCross) {
     b = c->grid2_fft.as<cplx<T>>();
   }
-  if (p.fast[0]) {
-    Stage st(c, FC_STAGE_COMBINE);
-    if (mode == kModeInterlaced)
-      launch_periodic_x<T, kModeInterlaced>(c, w, b);
-    else if (mode == kModeCross)
-      launch_periodic_x<T, kModeCross>(c, w, b);
-    else
-      launch_periodic_x<T, kModeAuto>(c, w, b);
-  } else {
-    // Unusual x length: separate x pass, combine, gather, pruned inverse x.
-    forward_x<T>(c, w);
-    if (mode == kModeInterlaced) forward_x<T>(c, c->work_b.as<cplx<T>>());
-    Stage st(c, FC_STAGE_COMBINE);
-    if (mode == kModeInterlaced) {
-      hipLaunchKernelGGL(k_interlace<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, w, b,
-                         p.n[0], p.n[1], (int64_t)p.ldc, c->ph[0].as<double2>(),
-                         c->ph[1].as<double2>(), c->ph[2].as<double2>());
-      FC_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(k_conj_mult<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, w,
-                       mode == kModeCross ? b : nullptr, nc);
-    FC_LAUNCH_CHECK();
-    const KSlots ks = kslots(c);
-    const int64_t npk = (int64_t)ks.ks0 * ks.ks1 * (ks.ks2 / 2 + 1);
-    hipLaunchKernelGGL(k_gather_pk<T>, dim3(grid1d(npk, 256, 65536)), dim3(256), 0, s,
-                       c->pk.as<T>(), w, ks, p.n[0], p.n[1], (int64_t)p.ldc);
-    FC_LAUNCH_CHECK();
-    const int64_t ldc = p.ldc;
-    ColView<T> v{w, 0, (int64_t)p.n[1] * ldc, ldc, c->q1.as<cplx<T>>(), 0,
-                 (int64_t)p.n[1] * ldc, ldc, 1, p.n[1], p.n2h, c->rshape, c->rshape / 2};
-    launch_cols<T, +1>(p.n[0], v, p.tile[0], p.tw0.as<cplx<T>>(), s);
-  }
+  power_and_pruned_inverse<T>(c, w, b, mode);
   switch (c->nell) {
     case 1:
-      periodic_tail<T, 1>(c);
+      periodic_hists<T, 1>(c);
       break;
     case 2:
-      periodic_tail<T, 2>(c);
+      periodic_hists<T, 2>(c);
       break;
     default:
-      periodic_tail<T, 3>(c);
+      periodic_hists<T, 3>(c);
   }
 }
 
@@ -801,63 +773,54 @@ template <typename T>
 void yamamoto_correlate(fc_corr* c, fc_grid* g1) {
   hipStream_t s = c->ctx->stream;
   const FftPlan& p = *c->plan;
-  const int64_t nc = p.nrows() * p.ldc;
   cplx<T>* w = c->work.as<cplx<T>>();
   const uint64_t ncells = (uint64_t)c->shape[0] * c->shape[1] * c->shape[2];
+  ensure_periodic_buffers<T>(c);
+  ensure_phase_tables(c);
   // Observer at the survey origin; cell centres 0.5 - observer + i
   // (correlator.h:381-402).
   double start[3];
   for (int i = 0; i < 3; ++i) start[i] = 0.5 - (-c->posmin[i] / c->cell);
+  const int rs = c->rshape;
   const int64_t nk = (int64_t)c->kshape[0] * c->kshape[1] * c->kshape[2];
-  const int64_t nr = (int64_t)c->rshape * c->rshape * c->rshape;
+  const int64_t nr = (int64_t)rs * rs * rs;
   c->rsub.reserve(sizeof(double) * nr);
   c->ksub.reserve(sizeof(double) * nk);
-  KMap km{p.n[0], p.n[1], (int64_t)p.ldc, c->kshape[0], c->kshape[1], c->kshape[2]};
-  RMap rm{p.n[0], p.n[1], p.n[2], 2 * (int64_t)p.ldc, c->rshape};
+  const KSlots ks = kslots(c);
   for (int ell = 0; ell <= c->maxell; ell += 2) {
     fill<double>(c->rsub.as<double>(), nr, 0.0, s);
     fill<double>(c->ksub.as<double>(), nk, 0.0, s);
     for (int m = -ell; m <= ell; ++m) {
       LoadYlm ld{ell, m, p.n[1], ylm_coeff(ell, m, 1.0), start[0], start[1], start[2]};
-      forward_fft<T>(c, g1, w, ld);
-      {
-        Stage st(c, FC_STAGE_COMBINE);
-        hipLaunchKernelGGL(k_conj_mult<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, w,
-                           c->grid2_fft.as<cplx<T>>(), nc);
-        FC_LAUNCH_CHECK();
-      }
-      {
-        Stage st(c, FC_STAGE_SUBGRID);
-        hipLaunchKernelGGL(k_acc_k<T>, dim3(grid1d(nk, 256, 65536)), dim3(256), 0, s,
-                           c->ksub.as<double>(), w, km, c->kx.as<double>(), c->ky.as<double>(),
-                           c->kz.as<double>(), c->wx.as<double>(), c->wy.as<double>(),
-                           c->wz.as<double>(), ell, m, ylm_coeff(ell, m, 4.0 * M_PI));
-        FC_LAUNCH_CHECK();
-      }
-      inverse_fft<T>(c, w);
-      {
-        Stage st(c, FC_STAGE_SUBGRID);
-        hipLaunchKernelGGL(k_acc_r<T>, dim3(grid1d(nr, 256, 65536)), dim3(256), 0, s,
-                           c->rsub.as<double>(), reinterpret_cast<const T*>(w), rm,
-                           c->rx.as<double>(), ell, m, ylm_coeff(ell, m, 4.0 * M_PI / ncells));
-        FC_LAUNCH_CHECK();
-      }
+      forward_zy<T>(c, g1, w, ld);
+      power_and_pruned_inverse<T>(c, w, c->grid2_fft.as<cplx<T>>(), kModeCross);
+      Stage st(c, FC_STAGE_SUBGRID);
+      hipLaunchKernelGGL(k_acc_k<T>, dim3(grid1d(nk, 256, 65536)), dim3(256), 0, s,
+                         c->ksub.as<double>(), c->pk.as<T>(), ks, c->kx.as<double>(),
+                         c->ky.as<double>(), c->kz.as<double>(), c->wx.as<double>(),
+                         c->wy.as<double>(), c->wz.as<double>(), ell, m,
+                         ylm_coeff(ell, m, 4.0 * M_PI));
+      FC_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_acc_r<T>, dim3(grid1d(nr, 256, 65536)), dim3(256), 0, s,
+                         c->rsub.as<double>(), c->xisub.as<T>(), rs, c->rx.as<double>(), ell, m,
+                         ylm_coeff(ell, m, 4.0 * M_PI / ncells));
+      FC_LAUNCH_CHECK();
     }
     {
       Stage st(c, FC_STAGE_HIST_R);
-      run_hist_sorted<1>(SubgridVals{c->rsub.as<double>()}, c->rbin, c->hs,
-                         c->r_hist.as<double>(), c->r_counts.as<int>(), ell / 2, ell / 2,
-                         ell / 2 + 1, s);
+      run_hist_static<1, double>(c->rsub.as<double>(), c->rbin.cells.as<uint32_t>(), nullptr,
+                                 c->rbin, c->hs, c->r_hist.as<double>(), c->r_counts.as<int>(),
+                                 ell / 2, ell / 2, ell / 2 + 1, s);
     }
     {
       Stage st(c, FC_STAGE_HIST_K);
-      run_hist_sorted<1>(SubgridVals{c->ksub.as<double>()}, c->kbin, c->hs,
-                         c->k_hist.as<double>(), c->k_counts.as<int>(), ell / 2, ell / 2,
-                         ell / 2 + 1, s);
+      run_hist_static<1, double>(c->ksub.as<double>(), c->kbin.cells.as<uint32_t>(), nullptr,
+                                 c->kbin, c->hs, c->k_hist.as<double>(), c->k_counts.as<int>(),
+                                 ell / 2, ell / 2, ell / 2 + 1, s);
     }
     if (ell == 0) {
-      const int o = c->rshape / 2;
-      const int64_t center = ((int64_t)o * c->rshape + o) * c->rshape + o;
+      const int o = rs / 2;
+      const int64_t center = ((int64_t)o * rs + o) * rs + o;
       FC_HIP(hipMemcpyAsync(c->zerolag.as<double>(), c->rsub.as<double>() + center,
                             sizeof(double), hipMemcpyDeviceToDevice, s));
     }

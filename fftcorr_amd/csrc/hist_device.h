@@ -93,64 +93,37 @@ __global__ void __launch_bounds__(256)
     part_counts[(size_t)blockIdx.x * nbins + i] = lcounts[i];
 }
 
-// hist[row0 + q][b] += sum over blocks (in order) of part_hist[blk][q][b];
-// counts[row][b] += sum of part_counts for every row in [crow0, crow1).
-template <int NV>
-__global__ void k_hist_final(const double* __restrict__ part_hist,
-                             const int* __restrict__ part_counts, int nblocks, int nbins,
-                             double* __restrict__ hist, int* __restrict__ counts, int row0,
-                             int crow0, int crow1) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < NV * nbins) {
-    double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += part_hist[(size_t)b * NV * nbins + i];
-    const int q = i / nbins, bin = i % nbins;
-    hist[(size_t)(row0 + q) * nbins + bin] += s;
-  }
-  if (i < nbins) {
-    int c = 0;
-    for (int b = 0; b < nblocks; ++b) c += part_counts[(size_t)b * nbins + i];
-    for (int r = crow0; r < crow1; ++r) counts[(size_t)r * nbins + i] += c;
-  }
-}
-
 // Cells of an [s0, s1, s2] subgrid grouped by histogram bin.  The bin of a
 // cell depends only on the subgrid axes (correlator.h:129-135, :180-186 and
 // histogram_list.h:57), so it is computed once on the host, in FP64 exactly
 // as the reference does, and the cells are stored sorted by bin (raster
-// order within a bin).  A wave then reads 64 consecutive entries -- almost
-// always one bin -- and the per-bin counts are known without counting.
+// order within a bin) with each entry's bin beside it.  A wave then reads 64
+// consecutive entries -- almost always one bin -- and the per-bin counts are
+// known without counting.
 struct BinnedCells {
   DeviceBuffer cells;        // uint32 linear index (i*s1 + j)*s2 + k, sorted by bin
-  DeviceBuffer start;        // int32 [nbins + 1]: first entry of each bin
+  DeviceBuffer bins;         // int32 bin of each entry
   DeviceBuffer counts;       // int32 [nbins]
   std::vector<int> h_counts;
   int64_t n = 0;             // entries (cells whose bin is in range)
   int nbins = 0;
+  int64_t tracked_pos = -1;  // entry position of the tracked cell (or -1)
 };
 
+// track_cell >= 0: record the entry position of that cell in tracked_pos.
 void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
                         const std::vector<double>& ay, const std::vector<double>& az,
-                        double binsize, int nbins);
+                        double binsize, int nbins, int64_t track_cell = -1);
 
-__device__ __forceinline__ int bin_of_entry(const int* __restrict__ start, int nbins,
-                                            int64_t pos) {
-  int lo = 0, hi = nbins - 1;  // largest b with start[b] <= pos
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if ((int64_t)start[mid] <= pos)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
-
-// Histogram over bin-sorted cells: F(cell, v) fills NV values for a cell.
-template <int NV, class F>
+// Histogram over bin-sorted entries: value x = vals[src[pos]] (FP64), the NV
+// sample values are wt[q][pos] * x (or x when wt is null).  When pos == zpos
+// the first value is also stored to *zval (the zero-lag cell).
+template <int NV, typename V>
 __global__ void __launch_bounds__(256)
-    k_hist_sorted(F f, const uint32_t* __restrict__ cells, const int* __restrict__ start,
-                  int64_t n, int nbins, double* __restrict__ part_hist) {
+    k_hist_static(const V* __restrict__ vals, const uint32_t* __restrict__ src,
+                  const int* __restrict__ binof, const double* __restrict__ wt, int64_t n,
+                  int nbins, double* __restrict__ part_hist, int64_t zpos,
+                  double* __restrict__ zval) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nw = blockDim.x / kWave;
   double* bins = reinterpret_cast<double*>(smem_raw);
@@ -170,8 +143,11 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = 0.0;
     if (pos < hi) {
-      bin = bin_of_entry(start, nbins, pos);
-      f((int64_t)cells[pos], v);
+      bin = binof[pos];
+      const double x = (double)vals[src[pos]];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) v[q] = wt ? wt[(int64_t)q * n + pos] * x : x;
+      if (pos == zpos) *zval = v[0];
     }
     wave_hist_add<NV>(bin, v, wb, lcounts, nbins);
   }
@@ -183,21 +159,40 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// hist rows += block partials (in block order); count rows += the static counts.
+// One block per (multipole, bin): hist[row0 + q][bin] += the block partials
+// summed in a fixed order (strided per thread, then a fixed tree); counts
+// rows [crow0, crow1) += static_counts[bin] or the summed block counts.
 template <int NV>
-__global__ void k_hist_final_static(const double* __restrict__ part_hist, int nblocks, int nbins,
-                                    const int* __restrict__ counts_in,
-                                    double* __restrict__ hist, int* __restrict__ counts,
-                                    int row0, int crow0, int crow1) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256)
+    k_hist_final(const double* __restrict__ part_hist, const int* __restrict__ part_counts,
+                 const int* __restrict__ static_counts, int nblocks, int nbins,
+                 double* __restrict__ hist, int* __restrict__ counts, int row0, int crow0,
+                 int crow1) {
+  __shared__ double sd[256 / kWave];
+  __shared__ unsigned long long su[256 / kWave];
+  const int i = blockIdx.x;
   if (i < NV * nbins) {
     double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += part_hist[(size_t)b * NV * nbins + i];
-    const int q = i / nbins, bin = i % nbins;
-    hist[(size_t)(row0 + q) * nbins + bin] += s;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+      s += part_hist[(size_t)b * NV * nbins + i];
+    s = block_sum(s, sd);
+    if (threadIdx.x == 0) {
+      const int q = i / nbins, bin = i % nbins;
+      hist[(size_t)(row0 + q) * nbins + bin] += s;
+    }
   }
-  if (i < nbins)
-    for (int r = crow0; r < crow1; ++r) counts[(size_t)r * nbins + i] += counts_in[i];
+  if (i < nbins) {
+    unsigned long long c = 0;
+    if (static_counts) {
+      c = (unsigned long long)static_counts[i];
+    } else {
+      for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+        c += (unsigned long long)part_counts[(size_t)b * nbins + i];
+      c = block_sum_u64(c, su);
+    }
+    if (threadIdx.x == 0)
+      for (int r = crow0; r < crow1; ++r) counts[(size_t)r * nbins + i] += (int)c;
+  }
 }
 
 // Scratch for one histogram pass.
@@ -209,23 +204,24 @@ struct HistScratch {
   }
 };
 
-template <int NV, class F>
-void run_hist_sorted(F f, const BinnedCells& bc, HistScratch& hs, double* hist, int* counts,
-                     int row0, int crow0, int crow1, hipStream_t s) {
+template <int NV, typename V>
+void run_hist_static(const V* vals, const uint32_t* src, const double* wt, const BinnedCells& bc,
+                     HistScratch& hs, double* hist, int* counts, int row0, int crow0, int crow1,
+                     hipStream_t s, int64_t zpos = -1, double* zval = nullptr) {
   const int nbins = bc.nbins;
   const int thr = hist_threads(NV, nbins);
   if (thr == 0) fail(FC_ERR_INVALID, "too many histogram bins (" + std::to_string(nbins) + ")");
   hs.reserve(NV, nbins);
   int nb = (int)std::min<int64_t>(kHistBlocks, (bc.n + thr - 1) / thr);
   if (nb < 1) nb = 1;
-  hipLaunchKernelGGL((k_hist_sorted<NV, F>), dim3(nb), dim3(thr), hist_lds_bytes(NV, nbins, thr),
-                     s, f, bc.cells.as<uint32_t>(), bc.start.as<int>(), bc.n, nbins,
-                     hs.part_hist.as<double>());
+  hipLaunchKernelGGL((k_hist_static<NV, V>), dim3(nb), dim3(thr), hist_lds_bytes(NV, nbins, thr),
+                     s, vals, src, bc.bins.as<int>(), wt, bc.n, nbins, hs.part_hist.as<double>(),
+                     zpos, zval);
   FC_LAUNCH_CHECK();
   const int tot = NV * nbins > nbins ? NV * nbins : nbins;
-  hipLaunchKernelGGL((k_hist_final_static<NV>), dim3((tot + 255) / 256), dim3(256), 0, s,
-                     hs.part_hist.as<double>(), nb, nbins, bc.counts.as<int>(), hist, counts,
-                     row0, crow0, crow1);
+  hipLaunchKernelGGL((k_hist_final<NV>), dim3(tot), dim3(256), 0, s, hs.part_hist.as<double>(),
+                     (const int*)nullptr, bc.counts.as<int>(), nb, nbins, hist, counts, row0,
+                     crow0, crow1);
   FC_LAUNCH_CHECK();
 }
 
@@ -241,9 +237,9 @@ void run_hist(F f, int64_t n, int nbins, HistScratch& hs, double* hist, int* cou
                      n, nbins, hs.part_hist.as<double>(), hs.part_counts.as<int>());
   FC_LAUNCH_CHECK();
   const int tot = NV * nbins > nbins ? NV * nbins : nbins;
-  hipLaunchKernelGGL((k_hist_final<NV>), dim3((tot + 255) / 256), dim3(256), 0, s,
-                     hs.part_hist.as<double>(), hs.part_counts.as<int>(), nb, nbins, hist,
-                     counts, row0, crow0, crow1);
+  hipLaunchKernelGGL((k_hist_final<NV>), dim3(tot), dim3(256), 0, s, hs.part_hist.as<double>(),
+                     hs.part_counts.as<int>(), (const int*)nullptr, nb, nbins, hist, counts,
+                     row0, crow0, crow1);
   FC_LAUNCH_CHECK();
 }
 
