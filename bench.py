@@ -10,10 +10,19 @@ FFTs, the interlacing combine, |F|^2, the k-space multipole histograms, the
 inverse FFT and the r-space multipole histograms, and leaves FP64 histograms
 on the device.
 
-Reported: correlations/s of the FP32-mixed form (the headline `value`),
-the FP64 plain form beside it, their ratio, the FP32-vs-FP64 error per
-multipole, p50/p99 latency, the dominant kernel's HBM roofline, and the FP64
-CPU oracle (scipy pocketfft, all host threads) timed on one correlation.
+Forms timed on the same grids:
+  * mixed, fused (the headline `value`): FP32 grids, forward FFTs and
+    products, FP64 pruned inverse FFT; FP64 Y_lm, windows, bins and sums;
+  * fp32, fused: the inverse in FP32 as well;
+  * fp64, fused: every stage in FP64;
+  * fp64, plain: the reference's stage sequence (full forward FFT,
+    conjugate product, per-multipole k extraction, full inverse FFT,
+    per-multipole r extraction) in FP64 on the GPU -- the denominator of
+    the mixed/plain ratio.
+Reported beside them: the error of each reduced-precision form against the
+FP64 fused result per multipole (floor 1e-3 max|l=0|), bit-exact counts,
+p50/p99 latency, the dominant kernel's HBM roofline, and the FP64 CPU oracle
+(scipy pocketfft, all host threads) timed on one correlation.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
 """
@@ -57,19 +66,20 @@ def make_grids(lib, ctx, pos, cfg, precision):
     return grids, ma_ms, st
 
 
-def stage_bytes(cfg, s_work, s_grid, kshape, rs):
+def stage_bytes(cfg, s_work, s_grid, kshape, rs, s_inv=None):
     """Algorithmic HBM bytes per launch of each stage: the data each stage
     must read and write once.  Complex half grid n^2 (n/2+1) elements of
     2 s_work bytes; real grid n^3 of s_grid; compact k-subgrid values
     ks0 ks1 (ks2/2+1) of s_work; pruned inverse arrays rs n (n/2+1) and
-    rs^2 (n/2+1) complex; r subgrid rs^3 real."""
+    rs^2 (n/2+1) complex and the r subgrid rs^3 real, of s_inv bytes."""
+    s_inv = s_inv or s_work
     n = cfg["n"]
     nh = n // 2 + 1
     half = n * n * nh * 2 * s_work
     pk = kshape[0] * kshape[1] * (kshape[2] // 2 + 1) * s_work
-    q1 = rs * n * nh * 2 * s_work
-    q2 = rs * rs * nh * 2 * s_work
-    xs = rs ** 3 * s_work
+    q1 = rs * n * nh * 2 * s_inv
+    q2 = rs * rs * nh * 2 * s_inv
+    xs = rs ** 3 * s_inv
     return {
         "fft_z_fwd": n ** 3 * s_grid + half,
         "fft_y_fwd": 2 * half,
@@ -81,12 +91,14 @@ def stage_bytes(cfg, s_work, s_grid, kshape, rs):
     }
 
 
-def run_form(lib, corr_mod, ctx, pos, cfg, precision, steps, warmup, profile):
+def run_form(lib, corr_mod, ctx, pos, cfg, precision, steps, warmup, profile,
+             algorithm="fused"):
     n = cfg["n"]
     cell = cfg["box"] / n
     grids, ma_ms, st = make_grids(lib, ctx, pos, cfg, precision)
     c = corr_mod.PeriodicCorrelator((n, n, n), cell, cfg["window"], cfg["rmax"], cfg["dr"],
                                     cfg["kmax"], cfg["dk"], cfg["maxell"], precision=precision)
+    c.algorithm = algorithm
     L = lib.lib()
     h = c._handle
     ga, gb = grids[0].handle, grids[1].handle
@@ -110,8 +122,7 @@ def run_form(lib, corr_mod, ctx, pos, cfg, precision, steps, warmup, profile):
     res = {k: np.array(v, copy=True) if isinstance(v, np.ndarray) else v
            for k, v in c.results().items()}
     out = dict(wall_s=wall, lat_ms=np.array(lat), stages=stages, res=res, ma_ms=ma_ms,
-               totw=st.totw, kshape=c.kshape, rshape=c.rshape)
-    deltas = None
+               totw=st.totw, kshape=c.kshape, rshape=c.rshape, steps=steps)
     return out, grids, c
 
 
@@ -164,27 +175,49 @@ def main():
     pos = np.ascontiguousarray(pos, np.float64)
     log("particles: %d" % pos.shape[0])
 
-    log("FP32-mixed form: %d warmup + %d timed steps" % (args.warmup, args.steps))
-    f32, g32, c32 = run_form(lib, corr_mod, ctx, pos, cfg, "fp32", args.steps, args.warmup,
-                             profile=True)
-    del g32
-    fp64_steps = args.fp64_steps or max(3, args.steps // 4)
-    log("FP64 plain form: 1 warmup + %d timed steps" % fp64_steps)
-    f64, g64, c64 = run_form(lib, corr_mod, ctx, pos, cfg, "fp64", fp64_steps, 1, profile=False)
+    side = args.fp64_steps or max(3, args.steps // 4)
+    log("mixed form (headline): %d warmup + %d timed steps" % (args.warmup, args.steps))
+    fm, gm, cm = run_form(lib, corr_mod, ctx, pos, cfg, "mixed", args.steps, args.warmup,
+                          profile=True)
+    del gm, cm
+    forms = {}
+    for name, prec, algo in (("fp32_fused", "fp32", "fused"), ("fp64_fused", "fp64", "fused"),
+                             ("fp64_plain", "fp64", "plain")):
+        log("%s form: 1 warmup + %d timed steps" % (name, side))
+        out, g, c = run_form(lib, corr_mod, ctx, pos, cfg, prec, side, 1, profile=False,
+                             algorithm=algo)
+        forms[name] = out
+        if name != "fp64_fused":
+            del g
+        else:
+            g64 = g
+        del c
 
-    value = args.steps / f32["wall_s"]
-    value64 = fp64_steps / f64["wall_s"]
-    err = {}
-    for key, tag in (("k_hist", "P"), ("r_hist", "xi")):
-        e = floor_err(f32["res"][key], f64["res"][key])
-        for q in range(e.shape[0]):
-            err["%s_%d" % (tag, 2 * q)] = float(e[q].max())
-    counts_equal = bool(np.array_equal(f32["res"]["k_counts"], f64["res"]["k_counts"])
-                        and np.array_equal(f32["res"]["r_counts"], f64["res"]["r_counts"]))
+    def rate(out):
+        return out["steps"] / out["wall_s"]
+
+    value = rate(fm)
+    ref64 = forms["fp64_fused"]["res"]
+
+    def errs(res):
+        err = {}
+        for key, tag in (("k_hist", "P"), ("r_hist", "xi")):
+            e = floor_err(res[key], ref64[key])
+            for q in range(e.shape[0]):
+                err["%s_%d" % (tag, 2 * q)] = float(e[q].max())
+        return err
+
+    def counts_equal(res):
+        return bool(np.array_equal(res["k_counts"], ref64["k_counts"])
+                    and np.array_equal(res["r_counts"], ref64["r_counts"]))
+
+    err = errs(fm["res"])
+    plain_vs_fused = max(float(floor_err(forms["fp64_plain"]["res"][k], ref64[k]).max())
+                         for k in ("k_hist", "r_hist"))
 
     # Roofline of the dominant stage (HIP events on the correlator's stream).
-    sb = stage_bytes(cfg, 4, 4, f32["kshape"], f32["rshape"][0])
-    st = {k: v for k, v in f32["stages"].items() if v[1] > 0}
+    sb = stage_bytes(cfg, 4, 4, fm["kshape"], fm["rshape"][0], s_inv=8)
+    st = {k: v for k, v in fm["stages"].items() if v[1] > 0}
     dom = max(st, key=lambda k: st[k][0])
     avg_ms = st[dom][0] / st[dom][1]
     achieved = sb[dom] / (avg_ms * 1e-3) / 1e9 if dom in sb else None
@@ -196,14 +229,23 @@ def main():
     if not args.no_cpu:
         log("CPU baseline: FP64 oracle on one correlation")
         dt, threads, ref = cpu_baseline(g64, cfg)
-        e_cpu = max(float(floor_err(f64["res"]["k_hist"], ref["k_hist"]).max()),
-                    float(floor_err(f64["res"]["r_hist"], ref["r_hist"]).max()))
+        e_cpu = max(float(floor_err(ref64["k_hist"], ref["k_hist"]).max()),
+                    float(floor_err(ref64["r_hist"], ref["r_hist"]).max()))
         cpu = {"value": 1.0 / dt, "unit": "correlations/s", "cores": threads, "kind": "port",
                "sample": "one interlaced 512^3 l<=4 correlation of the same grids, FP64 "
                          "numpy/scipy oracle (scipy.fft workers=%d), %.1f s" % (threads, dt),
                "gpu_fp64_vs_cpu_max_floor_rel_err": e_cpu}
 
-    lat = f32["lat_ms"]
+    lat = fm["lat_ms"]
+
+    def side_report(out):
+        r = {"value": rate(out), "ms_per_step": 1e3 * out["wall_s"] / out["steps"],
+             "p50_ms": float(np.percentile(out["lat_ms"], 50))}
+        return r
+
+    fp32 = side_report(forms["fp32_fused"])
+    fp32["max_floor_rel_err_vs_fp64"] = errs(forms["fp32_fused"]["res"])
+    fp32["counts_bit_exact"] = counts_equal(forms["fp32_fused"]["res"])
     line = {
         "metric": "correlations/s @512^3 l<=4, FP32-mixed vs FP64 plain at <=1e-5 rel err; p50/p99",
         "value": value,
@@ -211,7 +253,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * f32["wall_s"] / args.steps,
+        "ms_per_step": 1e3 * fm["wall_s"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -222,17 +264,21 @@ def main():
                         "8M Thomas points in a 2000 Mpc/h box, kmax 0.5 dk 0.005, "
                         "rmax 200 dr 5",
             "grid": cfg["n"], "maxell": cfg["maxell"], "particles": int(pos.shape[0]),
-            "k_subgrid": list(f32["kshape"]), "r_subgrid": list(f32["rshape"]),
-            "precision": "FP32 grids/FFTs/products, FP64 Y_lm/windows/bins/sums",
+            "k_subgrid": list(fm["kshape"]), "r_subgrid": list(fm["rshape"]),
+            "precision": "mixed: FP32 grids, forward FFTs and products; FP64 pruned inverse "
+                         "FFT, Y_lm, windows, bins and sums",
         },
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
-        "fp64_plain": {"value": value64, "ms_per_step": 1e3 * f64["wall_s"] / fp64_steps,
-                       "p50_ms": float(np.percentile(f64["lat_ms"], 50))},
-        "fp32_over_fp64": value / value64,
-        "fp32_vs_fp64_max_floor_rel_err": err,
-        "counts_bit_exact": counts_equal,
-        "mass_assign_ms_two_grids": f32["ma_ms"],
+        "max_floor_rel_err_vs_fp64": err,
+        "counts_bit_exact": counts_equal(fm["res"]),
+        "fp64_plain": side_report(forms["fp64_plain"]),
+        "fp64_fused": side_report(forms["fp64_fused"]),
+        "fp32_fused": fp32,
+        "mixed_over_fp64_plain": value / rate(forms["fp64_plain"]),
+        "mixed_over_fp64_fused": value / rate(forms["fp64_fused"]),
+        "fp64_plain_vs_fused_max_floor_rel_err": plain_vs_fused,
+        "mass_assign_ms_two_grids": fm["ma_ms"],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": (achieved / PEAK_HBM_GBS) if achieved else None,
                      "traffic": None},

@@ -24,6 +24,9 @@ FC_ERR_HIP = 5
 
 FC_FP64 = 0
 FC_FP32 = 1
+FC_MIXED = 2
+FC_ALGO_FUSED = 0
+FC_ALGO_PLAIN = 1
 FC_PERIODIC = 0
 FC_YAMAMOTO = 1
 
@@ -90,6 +93,7 @@ SIGNATURES = {
     "fc_corr_results": (ctypes.c_int, [_vp, _dp, _ip, _dp, _dp, _ip, _dp, _dp]),
     "fc_corr_last_time_ms": (ctypes.c_int, [_vp, _dp]),
     "fc_corr_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "fc_corr_set_algorithm": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fc_corr_stage_times": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "fc_hist_accumulate": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                           _dp, _dp, _i64, _ip, _dp]),
@@ -139,11 +143,25 @@ def iptr(a):
 
 
 def precision_code(precision):
-    if precision in (FC_FP64, "fp64", "float64", np.float64):
+    """Correlator precision: "fp64" (FC_FP64), "fp32" (FC_FP32) or "mixed"
+    (FC_MIXED: FP32 grids, forward FFTs and products, FP64 inverse)."""
+    if isinstance(precision, str):
+        key = precision.lower()
+    else:
+        key = precision
+    if key in (FC_FP64, "fp64", "float64", np.float64):
         return FC_FP64
-    if precision in (FC_FP32, "fp32", "float32", "mixed", np.float32):
+    if key in (FC_FP32, "fp32", "float32", np.float32):
         return FC_FP32
-    raise ValueError("precision must be 'fp64' or 'fp32', got %r" % (precision,))
+    if key in (FC_MIXED, "mixed"):
+        return FC_MIXED
+    raise ValueError("precision must be 'fp64', 'fp32' or 'mixed', got %r" % (precision,))
+
+
+def grid_precision_code(precision):
+    """Storage precision of a density grid: FP32 for "fp32" and "mixed"."""
+    code = precision_code(precision)
+    return FC_FP64 if code == FC_FP64 else FC_FP32
 
 
 class Context:
@@ -202,7 +220,7 @@ class DeviceGrid:
                  ctx=None):
         self.ctx = ctx or context()
         self.shape = tuple(int(s) for s in shape)
-        self.precision = precision_code(precision)
+        self.precision = grid_precision_code(precision)
         s = np.ascontiguousarray(self.shape, np.intc)
         pm = np.ascontiguousarray(posmin, np.float64)
         h = _vp()

@@ -4,8 +4,11 @@ cc/correlate/correlator.h:18-505) running on the GPU.
 Same constructors, ``from_grid_spec``, ``autocorrelate``, ``cross_correlate``,
 ``set_grid2`` and ``set_grid2_fft`` as the reference, returning
 (power_spectrum_table, correlation_table).  Additions:
-  * ``precision``: "fp64" (default; every stage in FP64 like the reference)
-    or "fp32" (FP32 grids/FFTs/products, FP64 sums and histograms);
+  * ``precision``: "fp64" (default; every stage in FP64 like the reference),
+    "fp32" (FP32 grids/FFTs/products, FP64 sums and histograms) or "mixed"
+    ("fp32" with the pruned inverse FFT in FP64);
+  * ``algorithm``: "fused" (default; fused x pass, pruned inverse) or
+    "plain" (the reference's stage sequence on the device, fp64/fp32 only);
   * ``device``: the GPU to run on;
   * ``PeriodicCorrelator.autocorrelate_interlaced(grid_a, grid_b)`` for a
     pair of grids assigned with shift 0 and 0.5 cells;
@@ -52,6 +55,7 @@ class BaseCorrelator:
         self.kshape = tuple(int(v) for v in ks)
         self._scratch = None
         self._have_results = False
+        self._algorithm = "fused"
 
     def __del__(self):
         h = getattr(self, "_handle", None)
@@ -143,6 +147,18 @@ class BaseCorrelator:
         t = ctypes.c_double()
         _lib.check(_lib.lib().fc_corr_last_time_ms(self._handle, ctypes.byref(t)))
         return t.value
+
+    @property
+    def algorithm(self):
+        return self._algorithm
+
+    @algorithm.setter
+    def algorithm(self, name):
+        codes = {"fused": _lib.FC_ALGO_FUSED, "plain": _lib.FC_ALGO_PLAIN}
+        if name not in codes:
+            raise ValueError("algorithm must be 'fused' or 'plain', got %r" % (name,))
+        _lib.check(_lib.lib().fc_corr_set_algorithm(self._handle, codes[name]))
+        self._algorithm = name
 
     def set_profiling(self, on=True):
         """Per-stage HIP-event timing of later correlate calls (resets totals)."""

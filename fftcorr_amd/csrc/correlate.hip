@@ -20,9 +20,11 @@
 // indices and every sum stay in FP64.
 #include <algorithm>
 #include <array>
+#include <type_traits>
 
 #include "fc_objects.h"
 #include "fft_plan.h"
+#include "fused_x.h"
 #include "hist_device.h"
 #include "ylm_device.h"
 
@@ -47,7 +49,9 @@ struct fc_corr {
   fc::BinnedCells kbin, rbin;                   // subgrid cells sorted by bin
   fc::DeviceBuffer ksrc, kw, rw;                // periodic: pk index + weights per entry
   // FFT plan + work arrays (complex rows of plan->ldc).
-  std::shared_ptr<fc::FftPlan> plan;
+  std::shared_ptr<fc::FftPlan> plan;      // forward precision
+  std::shared_ptr<fc::FftPlan> plan_inv;  // inverse precision (FP64 plan for FC_MIXED)
+  int algo = FC_ALGO_FUSED;
   fc::DeviceBuffer work, grid2_fft, work_b;
   bool have_grid2 = false;
   // Periodic: compact k-subgrid values, pruned inverse stages, r subgrid.
@@ -70,8 +74,6 @@ struct fc_corr {
 
 namespace fc {
 namespace {
-
-constexpr int kModeAuto = 0, kModeCross = 1, kModeInterlaced = 2;
 
 // ---- element-wise ----------------------------------------------------------
 // Complex rows of ldc (n2h used): out *= conj(in)
@@ -124,42 +126,6 @@ __global__ void k_from_c128(cplx<Tdst>* __restrict__ d, const double2* __restric
     const int64_t r = i / n2h;
     const int c = (int)(i - r * n2h);
     d[r * ldc + c] = cplx<Tdst>{(Tdst)s[i].x, (Tdst)s[i].y};
-  }
-}
-
-// ---- k-subgrid compact layout ---------------------------------------------
-// pk[sx][sy][c] holds Re P at the element with signed offsets
-// (sx - ox, sy - oy) along x, y and kz index c in [0, oz]: every element the
-// k subgrid reads (directly or reflected, fft_grid.cc:215-257).  Slots along
-// an axis of length n for index e: e + o (if e <= o) and e - n + o (if >= 0);
-// both exist only when the subgrid was truncated to n + 1 cells.
-struct KSlots {
-  int ks0, ks1, ks2;
-  __device__ __forceinline__ int oz() const { return ks2 / 2; }
-  __device__ __forceinline__ int64_t at(int sx, int sy, int c) const {
-    return ((int64_t)sx * ks1 + sy) * (ks2 / 2 + 1) + c;
-  }
-};
-
-__device__ __forceinline__ int slot_a(int e, int o) { return e <= o ? e + o : -1; }
-__device__ __forceinline__ int slot_b(int e, int n, int o) {
-  return e - n + o >= 0 ? e - n + o : -1;
-}
-
-// Writes Re P of element (e, y, c) to every k-subgrid slot that reads it.
-template <typename T>
-__device__ __forceinline__ void put_pk(T* pk, const KSlots& ks, int n0, int n1, int e, int y,
-                                       int c, T v) {
-  if (c > ks.oz()) return;
-  const int ox = ks.ks0 / 2, oy = ks.ks1 / 2;
-  const int sx[2] = {slot_a(e, ox), slot_b(e, n0, ox)};
-  const int sy[2] = {slot_a(y, oy), slot_b(y, n1, oy)};
-  for (int a = 0; a < 2; ++a) {
-    if (sx[a] < 0 || (a == 1 && sx[1] == sx[0])) continue;
-    for (int b = 0; b < 2; ++b) {
-      if (sy[b] < 0 || (b == 1 && sy[1] == sy[0])) continue;
-      pk[ks.at(sx[a], sy[b], c)] = v;
-    }
   }
 }
 
@@ -359,6 +325,29 @@ __global__ void k_acc_r(double* __restrict__ rsub, const T* __restrict__ xi, int
     const int j = (int)(t % rs), i = (int)(t / rs);
     const double y = ylm_eval(ell, m, rx[i], rx[j], rx[k], c, 1.0);
     rsub[idx] += y * (double)xi[idx];
+  }
+}
+
+// r-subgrid extraction from the full real correlation grid (rows of ldr
+// reals): rsub += Y_lm(r) c xi[(i - o) mod n0][(j - o) mod n1][(k - o) mod n2]
+// with o = rs / 2 (FftGrid::extract_submatrix, fft_grid.cc:182-210).
+template <typename T>
+__global__ void k_acc_r_full(double* __restrict__ rsub, const T* __restrict__ xi, int n0, int n1,
+                             int n2, int64_t ldr, int rs, const double* __restrict__ rx, int ell,
+                             int m, double c) {
+#pragma clang fp contract(off)
+  const int64_t n = (int64_t)rs * rs * rs;
+  const int o = rs / 2;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(idx % rs);
+    const int64_t t = idx / rs;
+    const int j = (int)(t % rs), i = (int)(t / rs);
+    const int a = ((i - o) % n0 + n0) % n0;
+    const int b = ((j - o) % n1 + n1) % n1;
+    const int d = ((k - o) % n2 + n2) % n2;
+    const double y = ylm_eval(ell, m, rx[i], rx[j], rx[k], c, 1.0);
+    rsub[idx] += y * (double)xi[((int64_t)a * n1 + b) * ldr + d];
   }
 }
 
@@ -582,17 +571,55 @@ void ensure_phase_tables(fc_corr* c) {
 
 KSlots kslots(const fc_corr* c) { return KSlots{c->kshape[0], c->kshape[1], c->kshape[2]}; }
 
-template <typename T>
+template <typename T, typename TI>
 void ensure_periodic_buffers(fc_corr* c) {
-  const FftPlan& p = *c->plan;
+  const FftPlan& pi = *c->plan_inv;
   const int rs = c->rshape;
   const int oz = c->kshape[2] / 2;
   c->pk.reserve(sizeof(T) * (size_t)c->kshape[0] * c->kshape[1] * (oz + 1));
-  c->q1.reserve(sizeof(cplx<T>) * (size_t)rs * p.n[1] * p.ldc);
-  c->q2.reserve(sizeof(cplx<T>) * (size_t)rs * rs * p.ldc);
-  c->xisub.reserve(sizeof(T) * (size_t)rs * rs * rs);
+  c->q1.reserve(sizeof(cplx<TI>) * (size_t)rs * pi.n[1] * pi.ldc);
+  c->q2.reserve(sizeof(cplx<TI>) * (size_t)rs * rs * pi.ldc);
+  c->xisub.reserve(sizeof(TI) * (size_t)rs * rs * rs);
 }
 
+// Register-chained fused x pass (power-of-two x lengths up to 512).
+template <typename T, typename TI, int MODE>
+void launch_periodic_x2(fc_corr* c, const cplx<T>* a, const cplx<T>* b) {
+  const FftPlan& p = *c->plan;
+  const FftPlan& pi = *c->plan_inv;
+  constexpr int TC = px_tc<T>();
+  const int tilesC = (p.n2h + TC - 1) / TC;
+  const int64_t nblocks = (int64_t)p.n[1] * tilesC;
+  if (nblocks > INT32_MAX) fail(FC_ERR_INVALID, "grid too large for the fused x pass");
+  const double2* ph[3] = {c->ph[0].as<double2>(), c->ph[1].as<double2>(),
+                          c->ph[2].as<double2>()};
+  const int rs = c->rshape;
+  const int offm = (rs / 2) % p.n[0];
+#define FC_PX2_CASE(Lv)                                                                       \
+  case Lv:                                                                                    \
+    hipLaunchKernelGGL((k_periodic_x2<T, TI, Lv, MODE>), dim3((unsigned)nblocks),             \
+                       dim3(px_threads<T, Lv>()), (px_lds_bytes<T, TI, Lv, MODE>()),           \
+                       c->ctx->stream, a, b, p.n[1], p.n2h, (int64_t)p.ldc, tilesC,            \
+                       p.tw0.as<cplx<T>>(), pi.tw0.as<cplx<TI>>(), ph[0], ph[1], ph[2],        \
+                       c->pk.as<T>(), kslots(c), c->q1.as<cplx<TI>>(), (int64_t)pi.ldc, rs,   \
+                       offm);                                                                 \
+    break;
+  switch (p.n[0]) {
+    FC_PX2_CASE(8)
+    FC_PX2_CASE(16)
+    FC_PX2_CASE(32)
+    FC_PX2_CASE(64)
+    FC_PX2_CASE(128)
+    FC_PX2_CASE(256)
+    FC_PX2_CASE(512)
+    default:
+      fail(FC_ERR_RUNTIME, "internal: no register-chained x kernel for this length");
+  }
+#undef FC_PX2_CASE
+  FC_LAUNCH_CHECK();
+}
+
+// LDS-staged fused x pass for the longer fast lengths (forward precision only).
 template <typename T, int MODE>
 void launch_periodic_x(fc_corr* c, cplx<T>* a, const cplx<T>* b) {
   const FftPlan& p = *c->plan;
@@ -611,13 +638,6 @@ void launch_periodic_x(fc_corr* c, cplx<T>* a, const cplx<T>* b) {
     break;                                                                                  \
   }
   switch (p.n[0]) {
-    FC_PX_CASE(8)
-    FC_PX_CASE(16)
-    FC_PX_CASE(32)
-    FC_PX_CASE(64)
-    FC_PX_CASE(128)
-    FC_PX_CASE(256)
-    FC_PX_CASE(512)
     FC_PX_CASE(1024)
     FC_PX_CASE(2048)
     FC_PX_CASE(4096)
@@ -636,57 +656,88 @@ void launch_periodic_x(fc_corr* c, cplx<T>* a, const cplx<T>* b) {
   FC_LAUNCH_CHECK();
 }
 
-// |F|^2 (or F conj(G)) -> compact k values pk, then the pruned inverse
-// (x lags in the fused kernel, then y and z) -> r subgrid xisub.
+template <typename T, typename TI, int MODE>
+void fused_x(fc_corr* c, cplx<T>* w, const cplx<T>* b) {
+  const int L = c->plan->n[0];
+  if (px_len_ok(L))
+    launch_periodic_x2<T, TI, MODE>(c, w, b);
+  else if constexpr (std::is_same<T, TI>::value)
+    launch_periodic_x<T, MODE>(c, w, b);
+  else
+    fail(FC_ERR_RUNTIME, "internal: fused x pass unavailable");
+}
+
+// Forward x pass(es), interlacing and the conjugate product over the whole
+// work grid w (the reference's execute_fft + multiply_with_conjugation).
 template <typename T>
-void power_and_pruned_inverse(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode) {
+void full_power(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode) {
   hipStream_t s = c->ctx->stream;
   const FftPlan& p = *c->plan;
   const int64_t nc = p.nrows() * p.ldc;
-  const int64_t ldc = p.ldc;
+  forward_x<T>(c, w);
+  if (mode == kModeInterlaced) forward_x<T>(c, const_cast<cplx<T>*>(b));
+  Stage st(c, FC_STAGE_COMBINE);
+  if (mode == kModeInterlaced) {
+    hipLaunchKernelGGL(k_interlace<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, w, b,
+                       p.n[0], p.n[1], (int64_t)p.ldc, c->ph[0].as<double2>(),
+                       c->ph[1].as<double2>(), c->ph[2].as<double2>());
+    FC_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_conj_mult<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, w,
+                     mode == kModeCross ? b : nullptr, nc);
+  FC_LAUNCH_CHECK();
+}
+
+template <typename T>
+void gather_pk(fc_corr* c, const cplx<T>* P) {
+  const FftPlan& p = *c->plan;
+  const KSlots ks = kslots(c);
+  const int64_t npk = (int64_t)ks.ks0 * ks.ks1 * (ks.ks2 / 2 + 1);
+  hipLaunchKernelGGL(k_gather_pk<T>, dim3(grid1d(npk, 256, 65536)), dim3(256), 0,
+                     c->ctx->stream, c->pk.as<T>(), P, ks, p.n[0], p.n[1], (int64_t)p.ldc);
+  FC_LAUNCH_CHECK();
+}
+
+// |F|^2 (or F conj(G)) -> compact k values pk, then the pruned inverse
+// (x lags in the fused kernel, then y and z, in the inverse precision TI)
+// -> r subgrid xisub.
+template <typename T, typename TI>
+void power_and_pruned_inverse(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode) {
+  hipStream_t s = c->ctx->stream;
+  const FftPlan& p = *c->plan;
+  const FftPlan& pi = *c->plan_inv;
+  const int64_t ldq = pi.ldc;
   const int rs = c->rshape, o = rs / 2;
-  if (p.fast[0]) {
+  const bool fused = px_len_ok(p.n[0]) || (p.fast[0] && std::is_same<T, TI>::value);
+  if (fused) {
     Stage st(c, FC_STAGE_COMBINE);
     if (mode == kModeInterlaced)
-      launch_periodic_x<T, kModeInterlaced>(c, w, b);
+      fused_x<T, TI, kModeInterlaced>(c, w, b);
     else if (mode == kModeCross)
-      launch_periodic_x<T, kModeCross>(c, w, b);
+      fused_x<T, TI, kModeCross>(c, w, b);
     else
-      launch_periodic_x<T, kModeAuto>(c, w, b);
+      fused_x<T, TI, kModeAuto>(c, w, b);
   } else {
-    // Unusual x length: separate x pass, combine, gather, pruned inverse x.
-    forward_x<T>(c, w);
-    if (mode == kModeInterlaced) forward_x<T>(c, const_cast<cplx<T>*>(b));
-    Stage st(c, FC_STAGE_COMBINE);
-    if (mode == kModeInterlaced) {
-      hipLaunchKernelGGL(k_interlace<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, w, b,
-                         p.n[0], p.n[1], ldc, c->ph[0].as<double2>(), c->ph[1].as<double2>(),
-                         c->ph[2].as<double2>());
-      FC_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(k_conj_mult<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, w,
-                       mode == kModeCross ? b : nullptr, nc);
-    FC_LAUNCH_CHECK();
-    const KSlots ks = kslots(c);
-    const int64_t npk = (int64_t)ks.ks0 * ks.ks1 * (ks.ks2 / 2 + 1);
-    hipLaunchKernelGGL(k_gather_pk<T>, dim3(grid1d(npk, 256, 65536)), dim3(256), 0, s,
-                       c->pk.as<T>(), w, ks, p.n[0], p.n[1], ldc);
-    FC_LAUNCH_CHECK();
-    ColView<T> v{w, 0, (int64_t)p.n[1] * ldc, ldc, c->q1.as<cplx<T>>(), 0,
-                 (int64_t)p.n[1] * ldc, ldc, 1, p.n[1], p.n2h, rs, o};
-    launch_cols<T, +1>(p.n[0], v, p.tile[0], p.tw0.as<cplx<T>>(), s);
+    // Other x lengths: separate x pass, combine, gather, pruned inverse x
+    // (reading the power grid in T, transforming in TI).
+    full_power<T>(c, w, b, mode);
+    gather_pk<T>(c, w);
+    Stage st(c, FC_STAGE_FFT_X_INV);
+    ColView<TI, T> v{w, 0, (int64_t)p.n[1] * p.ldc, (int64_t)p.ldc, c->q1.as<cplx<TI>>(), 0,
+                     (int64_t)p.n[1] * ldq, ldq, 1, p.n[1], p.n2h, rs, o};
+    launch_cols<TI, +1, T>(p.n[0], v, pi.tile[0], pi.tw0.as<cplx<TI>>(), s);
   }
   // Pruned inverse y pass: q1[sx][y][c] -> q2[sx][sy][c], only the rs lags.
   {
     Stage st(c, FC_STAGE_FFT_Y_INV);
-    ColView<T> v{c->q1.as<cplx<T>>(), (int64_t)p.n[1] * ldc, ldc, 0, c->q2.as<cplx<T>>(),
-                 (int64_t)rs * ldc, ldc, 0, rs, 1, p.n2h, rs, o};
-    launch_cols<T, +1>(p.n[1], v, p.tile[1], p.tw1.as<cplx<T>>(), s);
+    ColView<TI> v{c->q1.as<cplx<TI>>(), (int64_t)pi.n[1] * ldq, ldq, 0, c->q2.as<cplx<TI>>(),
+                  (int64_t)rs * ldq, ldq, 0, rs, 1, pi.n2h, rs, o};
+    launch_cols<TI, +1>(pi.n[1], v, pi.tile[1], pi.tw1.as<cplx<TI>>(), s);
   }
   // Pruned z C2R: rows of q2 -> r subgrid rows xi[sx][sy][sz].
   Stage st(c, FC_STAGE_FFT_Z_INV);
-  launch_rows_c2r<T, T>(p, c->q2.as<cplx<T>>(), p.ldc, c->xisub.as<T>(), rs, s,
-                        (int64_t)rs * rs, rs, o);
+  launch_rows_c2r<TI, TI>(pi, c->q2.as<cplx<TI>>(), ldq, c->xisub.as<TI>(), rs, s,
+                          (int64_t)rs * rs, rs, o);
 }
 
 // Builds the periodic histogram weights once per correlator.
@@ -726,7 +777,7 @@ void ensure_periodic_weights(fc_corr* c) {
   FC_HIP(hipStreamSynchronize(s));  // coef is freed on return
 }
 
-template <typename T, int NV>
+template <typename T, typename TI, int NV>
 void periodic_hists(fc_corr* c) {
   hipStream_t s = c->ctx->stream;
   ensure_periodic_weights<NV>(c);
@@ -736,16 +787,16 @@ void periodic_hists(fc_corr* c) {
                            c->hs, c->k_hist.as<double>(), c->k_counts.as<int>(), 0, 0, NV, s);
   }
   Stage st(c, FC_STAGE_HIST_R);
-  run_hist_static<NV, T>(c->xisub.as<T>(), c->rbin.cells.as<uint32_t>(), c->rw.as<double>(),
-                         c->rbin, c->hs, c->r_hist.as<double>(), c->r_counts.as<int>(), 0, 0, NV,
-                         s, c->rbin.tracked_pos, c->zerolag.as<double>());
+  run_hist_static<NV, TI>(c->xisub.as<TI>(), c->rbin.cells.as<uint32_t>(), c->rw.as<double>(),
+                          c->rbin, c->hs, c->r_hist.as<double>(), c->r_counts.as<int>(), 0, 0,
+                          NV, s, c->rbin.tracked_pos, c->zerolag.as<double>());
 }
 
-template <typename T>
+template <typename T, typename TI>
 void periodic_correlate(fc_corr* c, fc_grid* g1, int mode, fc_grid* gb) {
   const FftPlan& p = *c->plan;
   cplx<T>* w = c->work.as<cplx<T>>();
-  ensure_periodic_buffers<T>(c);
+  ensure_periodic_buffers<T, TI>(c);
   ensure_phase_tables(c);
   forward_zy<T>(c, g1, w, LoadCopy{});
   const cplx<T>* b = nullptr;
@@ -756,74 +807,183 @@ void periodic_correlate(fc_corr* c, fc_grid* g1, int mode, fc_grid* gb) {
   } else if (mode == kModeCross) {
     b = c->grid2_fft.as<cplx<T>>();
   }
-  power_and_pruned_inverse<T>(c, w, b, mode);
+  power_and_pruned_inverse<T, TI>(c, w, b, mode);
   switch (c->nell) {
     case 1:
-      periodic_hists<T, 1>(c);
+      periodic_hists<T, TI, 1>(c);
       break;
     case 2:
-      periodic_hists<T, 2>(c);
+      periodic_hists<T, TI, 2>(c);
       break;
     default:
-      periodic_hists<T, 3>(c);
+      periodic_hists<T, TI, 3>(c);
   }
 }
 
+// ---- the reference's stage sequence (FC_ALGO_PLAIN) -------------------------
+// Full inverse 3D FFT of the power grid in w, in place (real rows of 2 ldc).
 template <typename T>
+T* full_inverse(fc_corr* c, cplx<T>* w) {
+  const FftPlan& p = *c->plan;
+  hipStream_t s = c->ctx->stream;
+  {
+    Stage st(c, FC_STAGE_FFT_X_INV);
+    launch_cols<T, +1>(p.n[0], x_view<T>(p, w), p.tile[0], p.tw0.as<cplx<T>>(), s);
+  }
+  {
+    Stage st(c, FC_STAGE_FFT_Y_INV);
+    launch_cols<T, +1>(p.n[1], y_view<T>(p, w), p.tile[1], p.tw1.as<cplx<T>>(), s);
+  }
+  Stage st(c, FC_STAGE_FFT_Z_INV);
+  T* xr = reinterpret_cast<T*>(w);
+  launch_rows_c2r<T, T>(p, w, p.ldc, xr, 2 * (int64_t)p.ldc, s);
+  return xr;
+}
+
+// One multipole of the k subgrid: ksub = Y_lm(k) coef W^-1(k) Re P.
+template <typename T>
+void plain_k_multipole(fc_corr* c, int ell, int m, double coef, bool accumulate) {
+  hipStream_t s = c->ctx->stream;
+  const int64_t nk = (int64_t)c->kshape[0] * c->kshape[1] * c->kshape[2];
+  Stage st(c, FC_STAGE_SUBGRID);
+  if (!accumulate) fill<double>(c->ksub.as<double>(), nk, 0.0, s);
+  hipLaunchKernelGGL(k_acc_k<T>, dim3(grid1d(nk, 256, 65536)), dim3(256), 0, s,
+                     c->ksub.as<double>(), c->pk.as<T>(), kslots(c), c->kx.as<double>(),
+                     c->ky.as<double>(), c->kz.as<double>(), c->wx.as<double>(),
+                     c->wy.as<double>(), c->wz.as<double>(), ell, m, ylm_coeff(ell, m, coef));
+  FC_LAUNCH_CHECK();
+}
+
+// One multipole of the r subgrid from the full real correlation grid.
+template <typename T>
+void plain_r_multipole(fc_corr* c, const T* xr, int ell, int m, double coef, bool accumulate) {
+  hipStream_t s = c->ctx->stream;
+  const FftPlan& p = *c->plan;
+  const int rs = c->rshape;
+  const int64_t nr = (int64_t)rs * rs * rs;
+  Stage st(c, FC_STAGE_SUBGRID);
+  if (!accumulate) fill<double>(c->rsub.as<double>(), nr, 0.0, s);
+  hipLaunchKernelGGL(k_acc_r_full<T>, dim3(grid1d(nr, 256, 65536)), dim3(256), 0, s,
+                     c->rsub.as<double>(), xr, p.n[0], p.n[1], p.n[2], 2 * (int64_t)p.ldc, rs,
+                     c->rx.as<double>(), ell, m, ylm_coeff(ell, m, coef));
+  FC_LAUNCH_CHECK();
+}
+
+void hist_subgrids(fc_corr* c, int ell, bool k_side, bool r_side) {
+  hipStream_t s = c->ctx->stream;
+  if (r_side) {
+    Stage st(c, FC_STAGE_HIST_R);
+    run_hist_static<1, double>(c->rsub.as<double>(), c->rbin.cells.as<uint32_t>(), nullptr,
+                               c->rbin, c->hs, c->r_hist.as<double>(), c->r_counts.as<int>(),
+                               ell / 2, ell / 2, ell / 2 + 1, s);
+    if (ell == 0) {
+      const int rs = c->rshape, o = rs / 2;
+      const int64_t center = ((int64_t)o * rs + o) * rs + o;
+      FC_HIP(hipMemcpyAsync(c->zerolag.as<double>(), c->rsub.as<double>() + center,
+                            sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+  }
+  if (k_side) {
+    Stage st(c, FC_STAGE_HIST_K);
+    run_hist_static<1, double>(c->ksub.as<double>(), c->kbin.cells.as<uint32_t>(), nullptr,
+                               c->kbin, c->hs, c->k_hist.as<double>(), c->k_counts.as<int>(),
+                               ell / 2, ell / 2, ell / 2 + 1, s);
+  }
+}
+
+void ensure_subgrids(fc_corr* c) {
+  const int rs = c->rshape;
+  c->rsub.reserve(sizeof(double) * (size_t)rs * rs * rs);
+  c->ksub.reserve(sizeof(double) * (size_t)c->kshape[0] * c->kshape[1] * c->kshape[2]);
+}
+
+// PeriodicCorrelator::correlate_internal (correlator.h:273-357) stage by stage.
+template <typename T>
+void periodic_plain(fc_corr* c, fc_grid* g1, int mode, fc_grid* gb) {
+  const FftPlan& p = *c->plan;
+  cplx<T>* w = c->work.as<cplx<T>>();
+  ensure_periodic_buffers<T, T>(c);
+  ensure_phase_tables(c);
+  ensure_subgrids(c);
+  forward_zy<T>(c, g1, w, LoadCopy{});
+  const cplx<T>* b = nullptr;
+  if (mode == kModeInterlaced) {
+    c->work_b.reserve(p.complex_bytes());
+    forward_zy<T>(c, gb, c->work_b.as<cplx<T>>(), LoadCopy{});
+    b = c->work_b.as<cplx<T>>();
+  } else if (mode == kModeCross) {
+    b = c->grid2_fft.as<cplx<T>>();
+  }
+  full_power<T>(c, w, b, mode);
+  gather_pk<T>(c, w);
+  const uint64_t ncells = (uint64_t)c->shape[0] * c->shape[1] * c->shape[2];
+  const double k_rescale = c->cell * c->cell * c->cell / ncells;
+  const double r_rescale = (1.0 / ncells / ncells);
+  for (int ell = 0; ell <= c->maxell; ell += 2) {
+    plain_k_multipole<T>(c, ell, 0, std::sqrt((4.0 * M_PI) * (2 * ell + 1)) * k_rescale, false);
+    hist_subgrids(c, ell, true, false);
+  }
+  const T* xr = full_inverse<T>(c, w);
+  for (int ell = 0; ell <= c->maxell; ell += 2) {
+    plain_r_multipole<T>(c, xr, ell, 0, std::sqrt((4.0 * M_PI) * (2 * ell + 1)) * r_rescale,
+                         false);
+    hist_subgrids(c, ell, false, true);
+  }
+}
+
+// Correlator::correlate_internal (correlator.h:405-491) stage by stage.
+template <typename T>
+void yamamoto_plain(fc_corr* c, fc_grid* g1) {
+  const FftPlan& p = *c->plan;
+  cplx<T>* w = c->work.as<cplx<T>>();
+  const uint64_t ncells = (uint64_t)c->shape[0] * c->shape[1] * c->shape[2];
+  ensure_periodic_buffers<T, T>(c);
+  ensure_subgrids(c);
+  double start[3];
+  for (int i = 0; i < 3; ++i) start[i] = 0.5 - (-c->posmin[i] / c->cell);
+  for (int ell = 0; ell <= c->maxell; ell += 2) {
+    for (int m = -ell; m <= ell; ++m) {
+      LoadYlm ld{ell, m, p.n[1], ylm_coeff(ell, m, 1.0), start[0], start[1], start[2]};
+      forward_zy<T>(c, g1, w, ld);
+      full_power<T>(c, w, c->grid2_fft.as<cplx<T>>(), kModeCross);
+      gather_pk<T>(c, w);
+      plain_k_multipole<T>(c, ell, m, 4.0 * M_PI, m != -ell);
+      const T* xr = full_inverse<T>(c, w);
+      plain_r_multipole<T>(c, xr, ell, m, 4.0 * M_PI / ncells, m != -ell);
+    }
+    hist_subgrids(c, ell, true, true);
+  }
+}
+
+template <typename T, typename TI>
 void yamamoto_correlate(fc_corr* c, fc_grid* g1) {
   hipStream_t s = c->ctx->stream;
   const FftPlan& p = *c->plan;
   cplx<T>* w = c->work.as<cplx<T>>();
   const uint64_t ncells = (uint64_t)c->shape[0] * c->shape[1] * c->shape[2];
-  ensure_periodic_buffers<T>(c);
+  ensure_periodic_buffers<T, TI>(c);
   ensure_phase_tables(c);
+  ensure_subgrids(c);
   // Observer at the survey origin; cell centres 0.5 - observer + i
   // (correlator.h:381-402).
   double start[3];
   for (int i = 0; i < 3; ++i) start[i] = 0.5 - (-c->posmin[i] / c->cell);
   const int rs = c->rshape;
-  const int64_t nk = (int64_t)c->kshape[0] * c->kshape[1] * c->kshape[2];
   const int64_t nr = (int64_t)rs * rs * rs;
-  c->rsub.reserve(sizeof(double) * nr);
-  c->ksub.reserve(sizeof(double) * nk);
-  const KSlots ks = kslots(c);
   for (int ell = 0; ell <= c->maxell; ell += 2) {
-    fill<double>(c->rsub.as<double>(), nr, 0.0, s);
-    fill<double>(c->ksub.as<double>(), nk, 0.0, s);
     for (int m = -ell; m <= ell; ++m) {
       LoadYlm ld{ell, m, p.n[1], ylm_coeff(ell, m, 1.0), start[0], start[1], start[2]};
       forward_zy<T>(c, g1, w, ld);
-      power_and_pruned_inverse<T>(c, w, c->grid2_fft.as<cplx<T>>(), kModeCross);
+      power_and_pruned_inverse<T, TI>(c, w, c->grid2_fft.as<cplx<T>>(), kModeCross);
+      plain_k_multipole<T>(c, ell, m, 4.0 * M_PI, m != -ell);
       Stage st(c, FC_STAGE_SUBGRID);
-      hipLaunchKernelGGL(k_acc_k<T>, dim3(grid1d(nk, 256, 65536)), dim3(256), 0, s,
-                         c->ksub.as<double>(), c->pk.as<T>(), ks, c->kx.as<double>(),
-                         c->ky.as<double>(), c->kz.as<double>(), c->wx.as<double>(),
-                         c->wy.as<double>(), c->wz.as<double>(), ell, m,
-                         ylm_coeff(ell, m, 4.0 * M_PI));
-      FC_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_acc_r<T>, dim3(grid1d(nr, 256, 65536)), dim3(256), 0, s,
-                         c->rsub.as<double>(), c->xisub.as<T>(), rs, c->rx.as<double>(), ell, m,
-                         ylm_coeff(ell, m, 4.0 * M_PI / ncells));
+      if (m == -ell) fill<double>(c->rsub.as<double>(), nr, 0.0, s);
+      hipLaunchKernelGGL(k_acc_r<TI>, dim3(grid1d(nr, 256, 65536)), dim3(256), 0, s,
+                         c->rsub.as<double>(), c->xisub.as<TI>(), rs, c->rx.as<double>(), ell,
+                         m, ylm_coeff(ell, m, 4.0 * M_PI / ncells));
       FC_LAUNCH_CHECK();
     }
-    {
-      Stage st(c, FC_STAGE_HIST_R);
-      run_hist_static<1, double>(c->rsub.as<double>(), c->rbin.cells.as<uint32_t>(), nullptr,
-                                 c->rbin, c->hs, c->r_hist.as<double>(), c->r_counts.as<int>(),
-                                 ell / 2, ell / 2, ell / 2 + 1, s);
-    }
-    {
-      Stage st(c, FC_STAGE_HIST_K);
-      run_hist_static<1, double>(c->ksub.as<double>(), c->kbin.cells.as<uint32_t>(), nullptr,
-                                 c->kbin, c->hs, c->k_hist.as<double>(), c->k_counts.as<int>(),
-                                 ell / 2, ell / 2, ell / 2 + 1, s);
-    }
-    if (ell == 0) {
-      const int o = rs / 2;
-      const int64_t center = ((int64_t)o * rs + o) * rs + o;
-      FC_HIP(hipMemcpyAsync(c->zerolag.as<double>(), c->rsub.as<double>() + center,
-                            sizeof(double), hipMemcpyDeviceToDevice, s));
-    }
+    hist_subgrids(c, ell, true, true);
   }
 }
 
@@ -839,6 +999,22 @@ void set_grid2_device(fc_corr* c, fc_grid* g2) {
   c->have_grid2 = true;
 }
 
+template <typename T, typename TI>
+void run_kind(fc_corr* c, fc_grid* g1, fc_grid* gb, int mode) {
+  const bool plain = c->algo == FC_ALGO_PLAIN;
+  if (c->kind == FC_YAMAMOTO) {
+    if constexpr (std::is_same<T, TI>::value) {
+      if (plain) return yamamoto_plain<T>(c, g1);
+    }
+    yamamoto_correlate<T, TI>(c, g1);
+  } else {
+    if constexpr (std::is_same<T, TI>::value) {
+      if (plain) return periodic_plain<T>(c, g1, mode, gb);
+    }
+    periodic_correlate<T, TI>(c, g1, mode, gb);
+  }
+}
+
 void correlate(fc_corr* c, fc_grid* g1, fc_grid* gb, int mode) {
   DeviceGuard dg(c->ctx->device);
   hipStream_t s = c->ctx->stream;
@@ -848,27 +1024,26 @@ void correlate(fc_corr* c, fc_grid* g1, fc_grid* gb, int mode) {
     check_grid_shape(c, gb);
     join_streams(c, gb);
   }
+  if (c->kind == FC_YAMAMOTO) {
+    if (mode == kModeInterlaced)
+      fail(FC_ERR_INVALID, "interlacing applies to the periodic correlator");
+  } else if (mode == kModeCross && !c->have_grid2) {
+    fail(FC_ERR_RUNTIME, "grid2 FFT not set: call set_grid2 first");
+  }
   ensure_work(c);
   flush_stage_times(c);
   FC_HIP(hipEventRecord(c->ev0, s));
   reset_hists(c, s);
   if (c->kind == FC_YAMAMOTO) {
-    if (mode == kModeInterlaced)
-      fail(FC_ERR_INVALID, "interlacing applies to the periodic correlator");
     if (mode == kModeAuto) set_grid2_device(c, g1);  // Correlator::autocorrelate = cross(grid, grid)
     if (!c->have_grid2) fail(FC_ERR_RUNTIME, "grid2 FFT not set: call set_grid2 first");
-    if (c->prec == FC_FP64)
-      yamamoto_correlate<double>(c, g1);
-    else
-      yamamoto_correlate<float>(c, g1);
-  } else {
-    if (mode == kModeCross && !c->have_grid2)
-      fail(FC_ERR_RUNTIME, "grid2 FFT not set: call set_grid2 first");
-    if (c->prec == FC_FP64)
-      periodic_correlate<double>(c, g1, mode, gb);
-    else
-      periodic_correlate<float>(c, g1, mode, gb);
   }
+  if (c->prec == FC_FP64)
+    run_kind<double, double>(c, g1, gb, mode);
+  else if (c->prec == FC_FP32)
+    run_kind<float, float>(c, g1, gb, mode);
+  else
+    run_kind<float, double>(c, g1, gb, mode);
   FC_HIP(hipEventRecord(c->ev1, s));
   c->timed = true;
 }
@@ -892,7 +1067,7 @@ int fc_corr_create(fc_ctx* ctx, int kind, const int shape[3], double cell_size,
       fc::fail(FC_ERR_INVALID, "unknown window correction " + std::to_string(window_correct));
     if (maxell < 0 || maxell > 4)
       fc::fail(FC_ERR_INVALID, "Unsupported ell: maxell must be 0, 2 or 4");
-    if (precision != FC_FP64 && precision != FC_FP32)
+    if (precision != FC_FP64 && precision != FC_FP32 && precision != FC_MIXED)
       fc::fail(FC_ERR_INVALID, "unknown precision");
     fc::DeviceGuard dg(ctx->device);
     std::unique_ptr<fc_corr> c(new fc_corr());
@@ -912,7 +1087,8 @@ int fc_corr_create(fc_ctx* ctx, int kind, const int shape[3], double cell_size,
     c->nell = maxell / 2 + 1;
     c->prec = precision;
     fc::setup_geometry(c.get());
-    c->plan = fc::get_fft_plan(ctx->device, shape, precision);
+    c->plan = fc::get_fft_plan(ctx->device, shape, precision == FC_FP64 ? FC_FP64 : FC_FP32);
+    c->plan_inv = precision == FC_MIXED ? fc::get_fft_plan(ctx->device, shape, FC_FP64) : c->plan;
     FC_HIP(hipEventCreate(&c->ev0));
     FC_HIP(hipEventCreate(&c->ev1));
     *out = c.release();
@@ -1054,6 +1230,17 @@ int fc_corr_stage_times(fc_corr* c, double* ms, int64_t* launches) {
       if (ms) ms[i] = c->stage_ms[i];
       if (launches) launches[i] = c->stage_n[i];
     }
+  });
+}
+
+int fc_corr_set_algorithm(fc_corr* c, int algo) {
+  return fc::guard([&] {
+    if (!c) fc::fail(FC_ERR_INVALID, "null correlator");
+    if (algo != FC_ALGO_FUSED && algo != FC_ALGO_PLAIN)
+      fc::fail(FC_ERR_INVALID, "unknown algorithm " + std::to_string(algo));
+    if (algo == FC_ALGO_PLAIN && c->prec == FC_MIXED)
+      fc::fail(FC_ERR_INVALID, "the plain algorithm runs in fp64 or fp32, not mixed");
+    c->algo = algo;
   });
 }
 

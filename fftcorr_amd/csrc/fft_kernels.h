@@ -85,6 +85,28 @@ struct Bfly<4, DIR, T> {
   }
 };
 
+// x * c for an irrational constant c.  In FP32 the constant is split as
+// c = hi + lo (hi = c rounded to float) and the product formed as
+// fma(hi, x, lo * x): the rounding of c alone would scale every product by
+// the same relative error (-1.7e-8 for 1/sqrt(2)), a bias that does not
+// average out in the multipole sums; with the split only the final rounding
+// remains.  FP64 multiplies directly.
+template <typename T>
+struct ConstMul;
+template <>
+struct ConstMul<double> {
+  double c;
+  __host__ __device__ constexpr ConstMul(long double v) : c((double)v) {}
+  __device__ __forceinline__ double operator()(double x) const { return c * x; }
+};
+template <>
+struct ConstMul<float> {
+  float hi, lo;
+  __host__ __device__ constexpr ConstMul(long double v)
+      : hi((float)v), lo((float)(v - (long double)(float)v)) {}
+  __device__ __forceinline__ float operator()(float x) const { return __builtin_fmaf(hi, x, lo * x); }
+};
+
 template <int DIR, typename T>
 struct Bfly<8, DIR, T> {
   using C = cplx<T>;
@@ -93,11 +115,11 @@ struct Bfly<8, DIR, T> {
     C o[4] = {v[1], v[3], v[5], v[7]};
     Bfly<4, DIR, T>::run(e);
     Bfly<4, DIR, T>::run(o);
-    const T h = T(0.70710678118654752440084436210484903928);
+    constexpr ConstMul<T> h(0.70710678118654752440084436210484903928L);
     // W8^1 = h (1 + DIR i), W8^2 = DIR i, W8^3 = h (-1 + DIR i)
-    C o1 = C{h * (o[1].x - DIR * o[1].y), h * (o[1].y + DIR * o[1].x)};
+    C o1 = C{h(o[1].x - DIR * o[1].y), h(o[1].y + DIR * o[1].x)};
     C o2 = mul_i<DIR>(o[2]);
-    C o3 = C{h * (-o[3].x - DIR * o[3].y), h * (-o[3].y + DIR * o[3].x)};
+    C o3 = C{h(-o[3].x - DIR * o[3].y), h(-o[3].y + DIR * o[3].x)};
     v[0] = cadd(e[0], o[0]);
     v[4] = csub(e[0], o[0]);
     v[1] = cadd(e[1], o1);
@@ -113,13 +135,14 @@ template <int DIR, typename T>
 struct Bfly<3, DIR, T> {
   using C = cplx<T>;
   static __device__ __forceinline__ void run(C* v) {
-    const T s3 = T(0.86602540378443864676372317075293618347);  // sqrt(3)/2
+    constexpr ConstMul<T> s3(0.86602540378443864676372317075293618347L);  // sqrt(3)/2
     C t = cadd(v[1], v[2]);
     C d = mul_i<DIR>(csub(v[1], v[2]));
     C m = C{v[0].x - T(0.5) * t.x, v[0].y - T(0.5) * t.y};
     v[0] = cadd(v[0], t);
-    v[1] = C{m.x + s3 * d.x, m.y + s3 * d.y};
-    v[2] = C{m.x - s3 * d.x, m.y - s3 * d.y};
+    const T sx = s3(d.x), sy = s3(d.y);
+    v[1] = C{m.x + sx, m.y + sy};
+    v[2] = C{m.x - sx, m.y - sy};
   }
 };
 
@@ -233,9 +256,10 @@ __device__ __forceinline__ void fft_lds(cplx<T>* lds, const cplx<T>* __restrict_
 // the lags a separation subgrid needs).  In place when in == out.
 // grid = (G * tilesB, A), tilesB = ceil(B / tc); block = tc * (L / E);
 // LDS = L * tc complex.
-template <typename T, int L, int DIR>
+// The input may be stored in another precision Tin (converted on load).
+template <typename T, int L, int DIR, typename Tin = T>
 __global__ void __launch_bounds__(1024)
-    k_fft_cols(const cplx<T>* in, int64_t inA, int64_t inL, int64_t inG, cplx<T>* out,
+    k_fft_cols(const cplx<Tin>* in, int64_t inA, int64_t inL, int64_t inG, cplx<T>* out,
                int64_t outA, int64_t outL, int64_t outG, int B, int tilesB, int tc, int nout,
                int off, const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
@@ -244,7 +268,7 @@ __global__ void __launch_bounds__(1024)
   const int g = blockIdx.x / tilesB;
   const int b0 = (blockIdx.x - g * tilesB) * tc;
   const int a = blockIdx.y;
-  const C* src = in + (int64_t)a * inA + (int64_t)g * inG + b0;
+  const cplx<Tin>* src = in + (int64_t)a * inA + (int64_t)g * inG + b0;
   C* dst = out + (int64_t)a * outA + (int64_t)g * outG + b0;
   const int nthr = blockDim.x;
   const int tid = threadIdx.x;
@@ -258,7 +282,12 @@ __global__ void __launch_bounds__(1024)
     for (int q = 0; q < E; ++q) {
       const int idx = tid + q * nthr;
       const int e = idx / tc, c = idx - e * tc;
-      tmp[q] = c < ncol ? src[e * inL + c] : C{T(0), T(0)};
+      if (c < ncol) {
+        const cplx<Tin> x = src[e * inL + c];
+        tmp[q] = C{T(x.x), T(x.y)};
+      } else {
+        tmp[q] = C{T(0), T(0)};
+      }
     }
 #pragma unroll
     for (int q = 0; q < E; ++q) lds[tid + q * nthr] = tmp[q];
@@ -274,9 +303,9 @@ __global__ void __launch_bounds__(1024)
 
 // Direct DFT fallback, one column per block (any L up to LDS capacity); same
 // data layout and pruning as k_fft_cols.  grid = (G * B, A).
-template <typename T, int DIR>
+template <typename T, int DIR, typename Tin = T>
 __global__ void __launch_bounds__(256)
-    k_dft_cols(const cplx<T>* in, int64_t inA, int64_t inL, int64_t inG, cplx<T>* out,
+    k_dft_cols(const cplx<Tin>* in, int64_t inA, int64_t inL, int64_t inG, cplx<T>* out,
                int64_t outA, int64_t outL, int64_t outG, int B, int L, int nout, int off,
                const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
@@ -284,9 +313,12 @@ __global__ void __launch_bounds__(256)
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int g = blockIdx.x / B;
   const int b = blockIdx.x - g * B;
-  const C* src = in + (int64_t)blockIdx.y * inA + (int64_t)g * inG + b;
+  const cplx<Tin>* src = in + (int64_t)blockIdx.y * inA + (int64_t)g * inG + b;
   C* dst = out + (int64_t)blockIdx.y * outA + (int64_t)g * outG + b;
-  for (int e = threadIdx.x; e < L; e += blockDim.x) lds[e] = src[e * inL];
+  for (int e = threadIdx.x; e < L; e += blockDim.x) {
+    const cplx<Tin> x = src[e * inL];
+    lds[e] = C{T(x.x), T(x.y)};
+  }
   __syncthreads();
   for (int o = threadIdx.x; o < nout; o += blockDim.x) {
     const int k = nout == L ? o : ((o - off) % L + L) % L;

@@ -44,9 +44,9 @@ std::shared_ptr<FftPlan> get_fft_plan(int device, const int shape[3], int prec);
 
 // ---- launchers -------------------------------------------------------------
 // Column view [A][L][G][B] for k_fft_cols / k_dft_cols (see fft_kernels.h).
-template <typename T>
+template <typename T, typename Tin = T>
 struct ColView {
-  const cplx<T>* in;
+  const cplx<Tin>* in;
   int64_t inA, inL, inG;
   cplx<T>* out;
   int64_t outA, outL, outG;
@@ -54,8 +54,8 @@ struct ColView {
   int nout = 0, off = 0;  // nout == 0: all L outputs
 };
 
-template <typename T, int DIR>
-void launch_cols(int L, const ColView<T>& v, int tc, const cplx<T>* tw, hipStream_t s) {
+template <typename T, int DIR, typename Tin = T>
+void launch_cols(int L, const ColView<T, Tin>& v, int tc, const cplx<T>* tw, hipStream_t s) {
   const size_t cs = sizeof(cplx<T>);
   const int nout = v.nout ? v.nout : L;
 #define FC_COLS_CASE(Lv)                                                                  \
@@ -63,7 +63,7 @@ void launch_cols(int L, const ColView<T>& v, int tc, const cplx<T>* tw, hipStrea
     const int thr = tc * (Lv / FftCfg<Lv>::E);                                            \
     const int tilesB = (v.B + tc - 1) / tc;                                               \
     dim3 g((unsigned)(v.G * tilesB), v.A);                                                \
-    hipLaunchKernelGGL((k_fft_cols<T, Lv, DIR>), g, dim3(thr), (size_t)Lv * tc * cs, s,   \
+    hipLaunchKernelGGL((k_fft_cols<T, Lv, DIR, Tin>), g, dim3(thr), (size_t)Lv * tc * cs, s,   \
                        v.in, v.inA, v.inL, v.inG, v.out, v.outA, v.outL, v.outG, v.B,     \
                        tilesB, tc, nout, v.off, tw);                                      \
     break;                                                                                \
@@ -89,7 +89,7 @@ void launch_cols(int L, const ColView<T>& v, int tc, const cplx<T>* tw, hipStrea
     FC_COLS_CASE(3072)
     default: {
       dim3 g((unsigned)(v.G * v.B), v.A);
-      hipLaunchKernelGGL((k_dft_cols<T, DIR>), g, dim3(256), (size_t)L * cs, s, v.in, v.inA,
+      hipLaunchKernelGGL((k_dft_cols<T, DIR, Tin>), g, dim3(256), (size_t)L * cs, s, v.in, v.inA,
                          v.inL, v.inG, v.out, v.outA, v.outL, v.outG, v.B, L, nout, v.off, tw);
     }
   }

@@ -1,0 +1,290 @@
+// Fused periodic x pass: forward x FFT of one or two grids (after their z and
+// y passes), the interlacing combine, |F|^2 or F conj(G), the compact
+// k-subgrid values, the inverse x FFT and the pruned store of the x lags the
+// r subgrid reads -- one HBM read of each input and no write of the full
+// power grid.  Replaces, for the x axis, the sequence execute_fft ->
+// multiply_with_conjugation -> extract_fft_submatrix_c2r -> execute_ifft of
+// cc/correlate/correlator.h:290-334 with cc/grid/fft_grid.cc:125-257.
+#pragma once
+
+#include "fft_kernels.h"
+
+namespace fc {
+
+constexpr int kModeAuto = 0, kModeCross = 1, kModeInterlaced = 2;
+
+// ---- k-subgrid compact layout ---------------------------------------------
+// pk[sx][sy][c] holds Re P at the element with signed offsets
+// (sx - ox, sy - oy) along x, y and kz index c in [0, oz]: every element the
+// k subgrid reads (directly or reflected, fft_grid.cc:215-257).  Slots along
+// an axis of length n for index e: e + o (if e <= o) and e - n + o (if >= 0);
+// both exist only when the subgrid was truncated to n + 1 cells.
+struct KSlots {
+  int ks0, ks1, ks2;
+  __host__ __device__ __forceinline__ int oz() const { return ks2 / 2; }
+  __host__ __device__ __forceinline__ int64_t at(int sx, int sy, int c) const {
+    return ((int64_t)sx * ks1 + sy) * (ks2 / 2 + 1) + c;
+  }
+};
+
+__device__ __forceinline__ int slot_a(int e, int o) { return e <= o ? e + o : -1; }
+__device__ __forceinline__ int slot_b(int e, int n, int o) {
+  return e - n + o >= 0 ? e - n + o : -1;
+}
+
+// Writes Re P of element (e, y, c) to every k-subgrid slot that reads it.
+template <typename T>
+__device__ __forceinline__ void put_pk(T* pk, const KSlots& ks, int n0, int n1, int e, int y,
+                                       int c, T v) {
+  if (c > ks.oz()) return;
+  const int ox = ks.ks0 / 2, oy = ks.ks1 / 2;
+  const int sx[2] = {slot_a(e, ox), slot_b(e, n0, ox)};
+  const int sy[2] = {slot_a(y, oy), slot_b(y, n1, oy)};
+  for (int a = 0; a < 2; ++a) {
+    if (sx[a] < 0 || (a == 1 && sx[1] == sx[0])) continue;
+    for (int b = 0; b < 2; ++b) {
+      if (sy[b] < 0 || (b == 1 && sy[1] == sy[0])) continue;
+      pk[ks.at(sx[a], sy[b], c)] = v;
+    }
+  }
+}
+
+// ---- register-chained Stockham column FFT ---------------------------------
+// A tile holds TC columns of length L in LDS, element (e, c) at row
+// e ^ ((e >> 3) & 1): rows 8a..8a+7 swap neighbours when bit 3 of the row is
+// set, so the first-stage stores of threads t and t + 1 (rows 8t + r) fall in
+// different LDS banks.
+template <int TC>
+struct LaySwz {
+  __device__ __forceinline__ int operator()(int c, int e) const {
+    return (e ^ ((e >> 3) & 1)) * TC + c;
+  }
+};
+
+template <class R>
+struct RadixInfo;
+template <int R>
+struct RadixInfo<Radices<R>> {
+  static constexpr int first = R, last = R;
+};
+template <int R, int R2, int... Rest>
+struct RadixInfo<Radices<R, R2, Rest...>> {
+  static constexpr int first = R, last = RadixInfo<Radices<R2, Rest...>>::last;
+};
+
+// Thread t of a column keeps E values in registers.  Before the first stage
+// slot s holds element t + s L/E (the first radix equals E).  After the last
+// stage (radix Rl) slot s holds element t + out_m(s) L/E.
+template <int L>
+__host__ __device__ constexpr int out_m(int s) {
+  constexpr int E = FftCfg<L>::E;
+  constexpr int Rl = RadixInfo<typename FftCfg<L>::radices>::last;
+  return s / Rl + (s % Rl) * (E / Rl);
+}
+
+// One Stockham stage on NT tiles in lock-step (one twiddle load serves every
+// tile).  FIRST: the inputs are already in v (no LDS read).  LAST: the
+// outputs stay in v (no LDS write).  A stage that writes LDS after reading it
+// synchronises in between; a stage that writes LDS ends with a barrier.
+template <typename T, int L, int R, int NS, int DIR, int NT, bool FIRST, bool LAST, int TC>
+__device__ __forceinline__ void rstage(cplx<T> (&v)[NT][FftCfg<L>::E], cplx<T>* lds,
+                                       const cplx<T>* __restrict__ tw, int c, int t) {
+  using C = cplx<T>;
+  constexpr int E = FftCfg<L>::E;
+  constexpr int TCOL = L / E;
+  constexpr int NB = E / R;
+  constexpr int LR = L / R;
+  constexpr int TS = L * TC;
+  const LaySwz<TC> lay;
+  if (!FIRST) {
+#pragma unroll
+    for (int q = 0; q < NT; ++q)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[q][b * R + r] = lds[q * TS + lay(c, t + b * TCOL + r * LR)];
+    if (!LAST) __syncthreads();
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * TCOL;
+    const int k = j % NS;
+    if (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        C w = tw[r * k * (L / (NS * R))];
+        if (DIR > 0) w.y = -w.y;
+#pragma unroll
+        for (int q = 0; q < NT; ++q) v[q][b * R + r] = cmul(v[q][b * R + r], w);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NT; ++q) Bfly<R, DIR, T>::run(&v[q][b * R]);
+    if (!LAST) {
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int q = 0; q < NT; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[q * TS + lay(c, base + r * NS)] = v[q][b * R + r];
+    }
+  }
+  if (!LAST) __syncthreads();
+}
+
+template <typename T, int L, int DIR, int NT, int NS, bool FIRST, int TC>
+__device__ __forceinline__ void rstages(Radices<>, cplx<T> (&)[NT][FftCfg<L>::E], cplx<T>*,
+                                        const cplx<T>*, int, int) {}
+
+template <typename T, int L, int DIR, int NT, int NS, bool FIRST, int TC, int R, int... Rest>
+__device__ __forceinline__ void rstages(Radices<R, Rest...>, cplx<T> (&v)[NT][FftCfg<L>::E],
+                                        cplx<T>* lds, const cplx<T>* __restrict__ tw, int c,
+                                        int t) {
+  rstage<T, L, R, NS, DIR, NT, FIRST, sizeof...(Rest) == 0, TC>(v, lds, tw, c, t);
+  rstages<T, L, DIR, NT, NS * R, false, TC>(Radices<Rest...>{}, v, lds, tw, c, t);
+}
+
+// ---- the fused kernel -------------------------------------------------------
+// Columns per workgroup: 64 bytes of each row (8 FP32 or 4 FP64 complex).
+template <typename T>
+constexpr int px_tc() {
+  return 64 / (int)sizeof(cplx<T>);
+}
+// Lengths the fused kernel takes: power-of-two plans whose first radix is E.
+__host__ __device__ constexpr bool px_len_ok(int L) {
+  return L >= 8 && L <= 512 && (L & (L - 1)) == 0;
+}
+template <typename T, int L>
+constexpr int px_threads() {
+  return px_tc<T>() * (L / FftCfg<L>::E);
+}
+// Waves per SIMD the register budget is sized for: two workgroups per CU.
+template <typename T>
+constexpr int px_waves() {
+  return sizeof(T) == 4 ? 4 : 2;
+}
+template <typename T, typename TI, int L, int MODE>
+constexpr size_t px_lds_bytes() {
+  constexpr size_t fwd = (MODE == kModeInterlaced ? 2 : 1) * (size_t)L * px_tc<T>() * sizeof(cplx<T>);
+  constexpr size_t inv = (size_t)L * px_tc<T>() * sizeof(cplx<TI>);
+  return fwd > inv ? fwd : inv;
+}
+
+// One workgroup per (y row, tile of TC kz columns) of the work grid
+// [n0][n1][ldc]; tiles are dealt so that each of the 8 XCDs (which receive
+// consecutive workgroups in turn) runs a contiguous range of tiles and the two
+// tiles sharing a 128-byte line meet in one L2.
+//   A: grid a after its z and y passes (forward precision T).
+//   B: grid b likewise (interlaced) or the fully transformed grid 2 (cross).
+//   pk: compact k-subgrid values (T); q[sx][y][c] (row stride ldq, inverse
+//   precision TI): inverse x FFT at element (sx - off) mod L for every
+//   sx < rs (offm = off mod L; rs may exceed L, then elements repeat).
+// Forward in T; combine, inverse and q in TI (FP64 for the mixed form).
+template <typename T, typename TI, int L, int MODE>
+__global__ void __launch_bounds__((px_threads<T, L>()))
+__attribute__((amdgpu_waves_per_eu(px_waves<T>())))
+    k_periodic_x2(const cplx<T>* __restrict__ A, const cplx<T>* __restrict__ B, int n1, int n2h,
+                  int64_t ldc, int tilesC, const cplx<T>* __restrict__ tw,
+                  const cplx<TI>* __restrict__ twi, const double2* __restrict__ ph0,
+                  const double2* __restrict__ ph1, const double2* __restrict__ ph2,
+                  T* __restrict__ pk, KSlots ks, cplx<TI>* __restrict__ q, int64_t ldq, int rs,
+                  int offm) {
+  using C = cplx<T>;
+  using CI = cplx<TI>;
+  constexpr int E = FftCfg<L>::E;
+  constexpr int TC = px_tc<T>();
+  constexpr int TCOL = L / E;
+  constexpr int NT = MODE == kModeInterlaced ? 2 : 1;
+  using Rad = typename FftCfg<L>::radices;
+  static_assert(RadixInfo<Rad>::first == E, "first radix must equal E");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  CI* ldsi = reinterpret_cast<CI*>(smem_raw);
+
+  const int nb = gridDim.x;
+  const int q8 = nb >> 3, r8 = nb & 7;
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int tile = (xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8) + slot;
+  const int y = tile / tilesC;
+  const int c0 = (tile - y * tilesC) * TC;
+  const int ncol = min(TC, n2h - c0);
+  const int c = threadIdx.x % TC, t = threadIdx.x / TC;
+  const int cg = c0 + c;
+  const bool colok = c < ncol;
+  const int64_t sL = (int64_t)n1 * ldc;
+  const int64_t base = (int64_t)y * ldc + cg;
+
+  C v[NT][E];
+  C g[MODE == kModeCross ? E : 1];
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const int64_t o = base + (int64_t)(t + s * TCOL) * sL;
+    v[0][s] = colok ? A[o] : C{T(0), T(0)};
+    if (NT == 2) v[NT - 1][s] = colok ? B[o] : C{T(0), T(0)};
+  }
+  if (MODE == kModeCross) {
+#pragma unroll
+    for (int s = 0; s < E; ++s)
+      g[s] = colok ? B[base + (int64_t)(t + out_m<L>(s) * TCOL) * sL] : C{T(0), T(0)};
+  }
+  rstages<T, L, -1, NT, 1, true, TC>(Rad{}, v, lds, tw, c, t);
+
+  // Combine: slot s holds element e = t + out_m(s) L/E of both spectra.
+  const int oz = ks.oz();
+  const bool kcol = colok && cg <= oz;
+  const int ox = ks.ks0 / 2, oy = ks.ks1 / 2;
+  int sy0 = slot_a(y, oy), sy1 = slot_b(y, n1, oy);
+  if (sy1 == sy0) sy1 = -1;
+  const int64_t pstride = (int64_t)ks.ks1 * (oz + 1);
+  const int64_t prow0 = (int64_t)sy0 * (oz + 1) + cg, prow1 = (int64_t)sy1 * (oz + 1) + cg;
+  double2 pyc = make_double2(1.0, 0.0);
+  if (MODE == kModeInterlaced) pyc = cmul(ph1[y], ph2[colok ? cg : 0]);
+  CI u[1][E];
+  {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int e = t + out_m<L>(s) * TCOL;
+      const TI ax = (TI)v[0][s].x, ay = (TI)v[0][s].y;
+      CI p;
+      if (MODE == kModeInterlaced) {
+        const double2 phd = cmul(ph0[e], pyc);
+        const TI px = (TI)phd.x, py = (TI)phd.y;
+        const TI bx = (TI)v[NT - 1][s].x, by = (TI)v[NT - 1][s].y;
+        const TI fx = TI(0.5) * (ax + (bx * px - by * py));
+        const TI fy = TI(0.5) * (ay + (bx * py + by * px));
+        p = CI{fx * fx + fy * fy, TI(0)};
+      } else if (MODE == kModeCross) {
+        const TI gx = (TI)g[s].x, gy = (TI)g[s].y;
+        p = CI{ax * gx + ay * gy, ay * gx - ax * gy};
+      } else {
+        p = CI{ax * ax + ay * ay, TI(0)};
+      }
+      u[0][out_m<L>(s)] = p;
+      if (kcol) {
+        int sx0 = slot_a(e, ox), sx1 = slot_b(e, L, ox);
+        if (sx1 == sx0) sx1 = -1;
+        const T val = (T)p.x;
+        if (sx0 >= 0) {
+          if (sy0 >= 0) pk[sx0 * pstride + prow0] = val;
+          if (sy1 >= 0) pk[sx0 * pstride + prow1] = val;
+        }
+        if (sx1 >= 0) {
+          if (sy0 >= 0) pk[sx1 * pstride + prow0] = val;
+          if (sy1 >= 0) pk[sx1 * pstride + prow1] = val;
+        }
+      }
+    }
+  }
+  __syncthreads();  // the forward pass's last LDS reads precede the inverse's stores
+  rstages<TI, L, +1, 1, 1, true, TC>(Rad{}, u, ldsi, twi, c, t);
+  if (colok) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      int sx = t + out_m<L>(s) * TCOL + offm;
+      if (sx >= L) sx -= L;
+      for (; sx < rs; sx += L) q[((int64_t)sx * n1 + y) * ldq + cg] = u[0][s];
+    }
+  }
+}
+
+}  // namespace fc

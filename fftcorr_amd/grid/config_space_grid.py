@@ -70,7 +70,7 @@ class ConfigSpaceGrid:
         self._metadata = {}
         if metadata:
             self._metadata.update(metadata)
-        self._precision = _lib.precision_code(precision)
+        self._precision = _lib.grid_precision_code(precision)
         self._device = device
         self._host = np.zeros(self._shape, dtype=np.float64)
         self._dev = None

@@ -15,7 +15,8 @@
  *                        3 CIC, 4 CIC aliased (new)   cc/correlate/window_correct.h:8-12
  * Precision: FC_FP64 computes every stage in FP64 (the reference's Float);
  * FC_FP32 keeps grids, FFTs and products in FP32 with FP64 sums, Y_lm
- * normalisation and histogram accumulation.
+ * normalisation and histogram accumulation; FC_MIXED is FC_FP32 with the
+ * (pruned) inverse FFT of the power/cross spectrum in FP64.
  */
 #ifndef FFTCORR_HIP_H
 #define FFTCORR_HIP_H
@@ -36,6 +37,15 @@ extern "C" {
 
 #define FC_FP64 0
 #define FC_FP32 1
+#define FC_MIXED 2
+
+/* Correlation algorithm: FC_ALGO_FUSED (default) runs the fused x pass and
+ * the pruned inverse; FC_ALGO_PLAIN runs the reference's stage sequence
+ * (full forward FFT, conjugate product, k-subgrid extraction per multipole,
+ * full inverse FFT, r-subgrid extraction per multipole;
+ * cc/correlate/correlator.h:273-357, :405-491) on the device. */
+#define FC_ALGO_FUSED 0
+#define FC_ALGO_PLAIN 1
 
 #define FC_PERIODIC 0 /* PeriodicCorrelator, cc/correlate/correlator.h:255-357 */
 #define FC_YAMAMOTO 1 /* Correlator,         cc/correlate/correlator.h:360-505 */
@@ -150,6 +160,10 @@ int fc_corr_last_time_ms(fc_corr* corr, double* ms);
 #define FC_STAGE_SUBGRID 9   /* Yamamoto subgrid accumulation        */
 #define FC_NSTAGES 10
 int fc_corr_set_profiling(fc_corr* corr, int on);
+/* Selects FC_ALGO_FUSED or FC_ALGO_PLAIN for later calls (no reference
+ * counterpart: the reference has one algorithm).  FC_ALGO_PLAIN needs
+ * FC_FP64 or FC_FP32. */
+int fc_corr_set_algorithm(fc_corr* corr, int algo);
 int fc_corr_stage_times(fc_corr* corr, double* ms, int64_t* launches);
 
 /* ---- HistogramList: cc/histogram/histogram_list.h:10-67,

@@ -1,7 +1,11 @@
 """GPU PeriodicCorrelator / Correlator (cc/correlate/correlator.h) against the
-FP64 oracle.  Counts bit-exact.  Histogram values: FP64 plain within 1e-9 and
-FP32-mixed within 2e-5 of the oracle, relative to max(|ref|, 1e-3 max|ref_l0|)
-per bin."""
+FP64 oracle.  Counts bit-exact.  Histogram values relative to
+max(|ref|, 1e-3 max|ref_l0|) per bin: FP64 within 1e-9; FP32 and mixed (FP32
+forward, FP64 inverse) within 3e-5 -- the smooth test fields put most of
+xi_0 into a few small-r cells, where the three FP32 roundings of the
+forward passes alone (numpy float32 emulation of the same Stockham passes,
+and scipy's float32 FFT, on the same inputs) reach 1e-5 of the floor.  Both
+the fused algorithm and the reference's plain stage sequence."""
 import numpy as np
 import pytest
 
@@ -12,7 +16,9 @@ from oracle import fftcorr_oracle as O
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp64": 1e-9, "fp32": 2e-5}
+TOL = {"fp64": 1e-9, "fp32": 3e-5, "mixed": 3e-5}
+FORMS = [("fp64", "fused"), ("fp32", "fused"), ("mixed", "fused"), ("fp64", "plain"),
+         ("fp32", "plain")]
 
 
 def _field(shape, seed, smooth=True):
@@ -45,15 +51,26 @@ PERIODIC = [((16, 16, 16), 10.0, 0, 40.0, 10.0, 0.3, 0.05, 0),
             ((16, 16, 16), 10.0, 3, 40.0, 10.0, 0.4, 0.05, 4),
             ((16, 16, 16), 10.0, 4, 300.0, 10.0, 0.4, 0.05, 4),
             ((64, 64, 64), 7.8125, 0, 200.0, 5.0, 0.03, 0.01, 0),
-            ((9, 10, 11), 3.0, 2, 20.0, 2.0, 2.0, 0.1, 4)]
+            ((9, 10, 11), 3.0, 2, 20.0, 2.0, 2.0, 0.1, 4),
+            ((512, 8, 8), 4.0, 3, 60.0, 4.0, 0.5, 0.05, 4),
+            ((256, 8, 6), 4.0, 1, 40.0, 4.0, 0.5, 0.05, 2),
+            ((128, 12, 16), 5.0, 0, 40.0, 5.0, 0.4, 0.05, 4),
+            ((8, 8, 8), 5.0, 2, 20.0, 5.0, 0.6, 0.05, 4)]
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def _make(cls, args, precision, algorithm):
+    c = cls(*args, precision=precision)
+    c.algorithm = algorithm
+    return c
+
+
+@pytest.mark.parametrize("form", FORMS, ids=["-".join(f) for f in FORMS])
 @pytest.mark.parametrize("case", PERIODIC)
-def test_periodic_against_oracle(case, precision):
+def test_periodic_against_oracle(case, form):
+    precision, algorithm = form
     shape, cell, win, rmax, dr, kmax, dk, maxell = case
     delta = _field(shape, sum(shape) + maxell)
-    c = PeriodicCorrelator(shape, cell, win, rmax, dr, kmax, dk, maxell, precision=precision)
+    c = _make(PeriodicCorrelator, case, precision, algorithm)
     ps, xi = c.autocorrelate(delta)
     ref = O.PeriodicCorrelator(shape, cell, win, rmax, dr, kmax, dk, maxell)
     ref.fast_hist = True
@@ -62,8 +79,36 @@ def test_periodic_against_oracle(case, precision):
     assert set(ps.colnames) == {"k", "ps", "histogram", "count"}
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_periodic_cross_and_grid2_fft(precision):
+INTERLACED = [((32, 32, 32), 5.0, 3, 50.0, 5.0, 0.5, 0.02, 4),
+              ((64, 16, 24), 4.0, 4, 40.0, 4.0, 0.6, 0.05, 4),
+              ((24, 16, 20), 4.0, 3, 30.0, 3.0, 0.9, 0.05, 2),
+              ((512, 8, 8), 4.0, 3, 60.0, 4.0, 0.5, 0.05, 4),
+              ((16, 16, 16), 10.0, 3, 300.0, 10.0, 0.4, 0.05, 4)]
+
+
+@pytest.mark.parametrize("form", FORMS, ids=["-".join(f) for f in FORMS])
+@pytest.mark.parametrize("case", INTERLACED)
+def test_periodic_interlaced_against_oracle(case, form):
+    """F = (F_a + F_b exp(i pi (fx/n0 + fy/n1 + fz/n2))) / 2 from two grids."""
+    precision, algorithm = form
+    shape = case[0]
+    a = _field(shape, 3 + sum(shape))
+    b = _field(shape, 4 + sum(shape))
+    ref = O.PeriodicCorrelator(*case)
+    ref.fast_hist = True
+    want = ref.autocorrelate_interlaced(a, b)
+    c = _make(PeriodicCorrelator, case, precision, algorithm)
+    ga = ConfigSpaceGrid(shape, [0, 0, 0], [case[1] * n for n in shape], precision=precision)
+    gb = ConfigSpaceGrid(shape, [0, 0, 0], [case[1] * n for n in shape], precision=precision)
+    np.copyto(ga.data, a)
+    np.copyto(gb.data, b)
+    c.autocorrelate_interlaced(ga, gb)
+    _compare(c.results(), want, precision)
+
+
+@pytest.mark.parametrize("form", FORMS, ids=["-".join(f) for f in FORMS])
+def test_periodic_cross_and_grid2_fft(form):
+    precision, algorithm = form
     shape = (32, 16, 24)
     a = _field(shape, 1)
     b = _field(shape, 2)
@@ -71,10 +116,10 @@ def test_periodic_cross_and_grid2_fft(precision):
     ref = O.PeriodicCorrelator(*args)
     ref.fast_hist = True
     want = ref.cross_correlate(a, b)
-    c = PeriodicCorrelator(*args, precision=precision)
+    c = _make(PeriodicCorrelator, args, precision, algorithm)
     c.cross_correlate(a, b)
     _compare(c.results(), want, precision)
-    c2 = PeriodicCorrelator(*args, precision=precision)
+    c2 = _make(PeriodicCorrelator, args, precision, algorithm)
     c2.set_grid2_fft(np.fft.rfftn(b))
     c2.cross_correlate(a)
     _compare(c2.results(), want, precision)
@@ -85,12 +130,13 @@ YAMAMOTO = [((16, 16, 16), 10.0, (-300.0, 50.0, 20.0), 0, 40.0, 10.0, 0.3, 0.05,
             ((32, 32, 32), 5.0, (500.0, 500.0, 500.0), 1, 60.0, 5.0, 0.3, 0.03, 4)]
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("form", FORMS, ids=["-".join(f) for f in FORMS])
 @pytest.mark.parametrize("case", YAMAMOTO)
-def test_yamamoto_against_oracle(case, precision):
+def test_yamamoto_against_oracle(case, form):
+    precision, algorithm = form
     shape, cell, posmin, win, rmax, dr, kmax, dk, maxell = case
     delta = _field(shape, sum(shape), smooth=True)
-    c = Correlator(shape, cell, posmin, win, rmax, dr, kmax, dk, maxell, precision=precision)
+    c = _make(Correlator, case, precision, algorithm)
     c.autocorrelate(delta)
     ref = O.Correlator(shape, cell, posmin, win, rmax, dr, kmax, dk, maxell)
     ref.fast_hist = True
@@ -134,5 +180,12 @@ def test_constructor_errors():
     c = PeriodicCorrelator((8, 8, 8), 1.0, 0, 5.0, 1.0, 0.5, 0.1, 0)
     with pytest.raises(ValueError):
         c.autocorrelate(np.zeros((8, 8, 9)))
+    with pytest.raises(ValueError):
+        c.algorithm = "fast"
+    with pytest.raises(ValueError):
+        PeriodicCorrelator((8, 8, 8), 1.0, 0, 5.0, 1.0, 0.5, 0.1, 0, precision="fp16")
+    cm = PeriodicCorrelator((8, 8, 8), 1.0, 0, 5.0, 1.0, 0.5, 0.1, 0, precision="mixed")
+    with pytest.raises(ValueError):
+        cm.algorithm = "plain"
     with pytest.raises(RuntimeError):
         c.cross_correlate(np.zeros((8, 8, 8)))  # no grid2 yet
