@@ -157,6 +157,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="time the mixed form only (for profiler runs); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the correlation path here is single-GPU "
@@ -180,6 +182,9 @@ def main():
     fm, gm, cm = run_form(lib, corr_mod, ctx, pos, cfg, "mixed", args.steps, args.warmup,
                           profile=True)
     del gm, cm
+    if args.headline_only:
+        log("mixed form: %.4f ms/correlation" % (1e3 * fm["wall_s"] / args.steps))
+        return
     forms = {}
     for name, prec, algo in (("fp32_fused", "fp32", "fused"), ("fp64_fused", "fp64", "fused"),
                              ("fp64_plain", "fp64", "plain")):

@@ -246,6 +246,317 @@ __device__ __forceinline__ void fft_lds(cplx<T>* lds, const cplx<T>* __restrict_
   run_stages<T, L, FftCfg<L>::E, DIR, 1>(typename FftCfg<L>::radices{}, lds, tw, c, t, lay);
 }
 
+// ---- register-chained Stockham passes -------------------------------------
+// For power-of-two lengths (first radix = E) thread t keeps E values of its
+// column or row in registers: before the first stage slot s holds element
+// t + s L/E, after the last stage (radix Rl) slot s holds element
+// t + out_m(s) L/E.  The first stage therefore reads global memory directly
+// and the last stage's outputs go straight to global memory (or to the next
+// fused step): two LDS passes fewer than the LDS-staged form above.
+template <class R>
+struct RadixInfo;
+template <int R>
+struct RadixInfo<Radices<R>> {
+  static constexpr int first = R, last = R;
+};
+template <int R, int R2, int... Rest>
+struct RadixInfo<Radices<R, R2, Rest...>> {
+  static constexpr int first = R, last = RadixInfo<Radices<R2, Rest...>>::last;
+};
+
+__host__ __device__ constexpr bool rc_len_ok(int L) {
+  return L >= 8 && L <= 512 && (L & (L - 1)) == 0;
+}
+
+template <int L>
+__host__ __device__ constexpr int out_m(int s) {
+  constexpr int E = FftCfg<L>::E;
+  constexpr int Rl = RadixInfo<typename FftCfg<L>::radices>::last;
+  return s / Rl + (s % Rl) * (E / Rl);
+}
+
+// Column tiles: TC columns of length L, element (e, c) at row
+// e ^ ((e >> 3) & 1): rows 8a..8a+7 swap neighbours when bit 3 of the row is
+// set, so the first-stage stores of threads t and t + 1 (rows 8t + r) fall in
+// different LDS banks when a row is narrower than the 128 bytes of banks.
+template <int TC>
+struct LaySwz {
+  __device__ __forceinline__ int operator()(int c, int e) const {
+    return (e ^ ((e >> 3) & 1)) * TC + c;
+  }
+};
+// Row tiles: row c of rs slots, element e at e + e / 8 (one pad slot per 8),
+// which spreads the stride-8 first-stage stores over all banks.
+struct LayRowPad {
+  int rs;
+  __device__ __forceinline__ int operator()(int c, int e) const { return c * rs + e + (e >> 3); }
+};
+__host__ __device__ constexpr int row_pad_len(int M) { return M + M / 8 + 1; }
+
+// One Stockham stage on NT tiles in lock-step (one twiddle load serves every
+// tile; tile q starts ts elements after tile q - 1).  FIRST: the inputs are
+// already in v (no LDS read).  LAST: the outputs stay in v (no LDS write).  A
+// stage that reads and then writes LDS synchronises in between; a stage that
+// writes LDS ends with a barrier.  A FIRST stage writes without a barrier
+// before it: the caller makes sure no thread still reads the LDS it writes.
+template <typename T, int L, int R, int NS, int DIR, int NT, bool FIRST, bool LAST, class Lay>
+__device__ __forceinline__ void rstage(cplx<T> (&v)[NT][FftCfg<L>::E], cplx<T>* lds, int ts,
+                                       const cplx<T>* __restrict__ tw, int c, int t, Lay lay) {
+  using C = cplx<T>;
+  constexpr int E = FftCfg<L>::E;
+  constexpr int TCOL = L / E;
+  constexpr int NB = E / R;
+  constexpr int LR = L / R;
+  if (!FIRST) {
+#pragma unroll
+    for (int q = 0; q < NT; ++q)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[q][b * R + r] = lds[q * ts + lay(c, t + b * TCOL + r * LR)];
+    if (!LAST) __syncthreads();
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * TCOL;
+    const int k = j % NS;
+    if (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        C w = tw[r * k * (L / (NS * R))];
+        if (DIR > 0) w.y = -w.y;
+#pragma unroll
+        for (int q = 0; q < NT; ++q) v[q][b * R + r] = cmul(v[q][b * R + r], w);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NT; ++q) Bfly<R, DIR, T>::run(&v[q][b * R]);
+    if (!LAST) {
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int q = 0; q < NT; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[q * ts + lay(c, base + r * NS)] = v[q][b * R + r];
+    }
+  }
+  if (!LAST) __syncthreads();
+}
+
+template <typename T, int L, int DIR, int NT, int NS, bool FIRST, class Lay>
+__device__ __forceinline__ void rstages(Radices<>, cplx<T> (&)[NT][FftCfg<L>::E], cplx<T>*, int,
+                                        const cplx<T>*, int, int, Lay) {}
+
+template <typename T, int L, int DIR, int NT, int NS, bool FIRST, class Lay, int R, int... Rest>
+__device__ __forceinline__ void rstages(Radices<R, Rest...>, cplx<T> (&v)[NT][FftCfg<L>::E],
+                                        cplx<T>* lds, int ts, const cplx<T>* __restrict__ tw,
+                                        int c, int t, Lay lay) {
+  rstage<T, L, R, NS, DIR, NT, FIRST, sizeof...(Rest) == 0>(v, lds, ts, tw, c, t, lay);
+  rstages<T, L, DIR, NT, NS * R, false>(Radices<Rest...>{}, v, lds, ts, tw, c, t, lay);
+}
+
+// XCD-aware block order: the hardware deals consecutive workgroups to the 8
+// XCDs in turn; give each XCD a contiguous range of tiles so that tiles
+// sharing cache lines meet in one L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  const int q8 = nb >> 3, r8 = nb & 7;
+  const int xcd = bid & 7, slot = bid >> 3;
+  return (xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8) + slot;
+}
+
+// Register-chained strided column pass over [A][L][G][B] (see k_fft_cols),
+// TC columns per workgroup, one launch of A * G * ceil(B / TC) workgroups.
+// Output o < nout holds element (o - off) mod L (offm = off mod L; nout may
+// exceed L).  In place when in == out: every workgroup reads its whole tile
+// before writing it.
+template <typename T, int L, int DIR, int TC, typename Tin = T>
+__global__ void __launch_bounds__(TC*(L / FftCfg<L>::E))
+    k_fft_cols_rc(const cplx<Tin>* in, int64_t inA, int64_t inL, int64_t inG, cplx<T>* out,
+                  int64_t outA, int64_t outL, int64_t outG, int B, int tilesB, int G, int nout,
+                  int offm, const cplx<T>* __restrict__ tw) {
+  using C = cplx<T>;
+  constexpr int E = FftCfg<L>::E;
+  constexpr int TCOL = L / E;
+  using Rad = typename FftCfg<L>::radices;
+  static_assert(RadixInfo<Rad>::first == E, "first radix must equal E");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int ag = tile / tilesB;
+  const int b0 = (tile - ag * tilesB) * TC;
+  const int a = ag / G, g = ag - a * G;
+  const int c = threadIdx.x % TC, t = threadIdx.x / TC;
+  const bool colok = b0 + c < B;
+  const cplx<Tin>* src = in + (int64_t)a * inA + (int64_t)g * inG + b0 + c;
+  C* dst = out + (int64_t)a * outA + (int64_t)g * outG + b0 + c;
+  C v[1][E];
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    if (colok) {
+      const cplx<Tin> x = src[(int64_t)(t + s * TCOL) * inL];
+      v[0][s] = C{T(x.x), T(x.y)};
+    } else {
+      v[0][s] = C{T(0), T(0)};
+    }
+  }
+  rstages<T, L, DIR, 1, 1, true>(Rad{}, v, lds, 0, tw, c, t, LaySwz<TC>{});
+  if (!colok) return;
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    int o = t + out_m<L>(s) * TCOL + offm;
+    if (o >= L) o -= L;
+    for (; o < nout; o += L) dst[(int64_t)o * outL] = v[0][s];
+  }
+}
+
+// Loads the real sample pair (2e, 2e + 1) of a row as one vector access.
+template <typename Tin>
+struct PairOf;
+template <>
+struct PairOf<float> {
+  using type = float2;
+};
+template <>
+struct PairOf<double> {
+  using type = double2;
+};
+
+// Register-chained R2C rows: a real row of 2M samples packed as M complex
+// values, a length-M FFT, then the Hermitian post-process (see
+// k_fft_rows_r2c).  Block = tr rows x M/E threads, LDS tr row_pad_len(M).
+template <typename T, typename Tin, int M, class Loader>
+__global__ void __launch_bounds__(1024)
+    k_fft_rows_r2c_rc(const Tin* in, int64_t ldi, cplx<T>* out, int64_t ldo, int64_t nrows,
+                      int tr, const cplx<T>* __restrict__ twM, const cplx<T>* __restrict__ twN,
+                      Loader ld) {
+  using C = cplx<T>;
+  constexpr int E = FftCfg<M>::E;
+  constexpr int TCOL = M / E;
+  constexpr int RS = row_pad_len(M);
+  using Rad = typename FftCfg<M>::radices;
+  static_assert(RadixInfo<Rad>::first == E, "first radix must equal E");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  const LayRowPad lay{RS};
+  const int c = threadIdx.x / TCOL, t = threadIdx.x % TCOL;
+  const int64_t row = (int64_t)blockIdx.x * tr + c;
+  const bool rowok = row < nrows;
+  C v[1][E];
+  {
+    using P = typename PairOf<Tin>::type;
+    const P* src = reinterpret_cast<const P*>(in + row * ldi);
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int e = t + s * TCOL;
+      C z{T(0), T(0)};
+      if (rowok) {
+        const P p = src[e];
+        z.x = T(ld((double)p.x, row, 2 * e));
+        z.y = T(ld((double)p.y, row, 2 * e + 1));
+      }
+      v[0][s] = z;
+    }
+  }
+  rstages<T, M, -1, 1, 1, true>(Rad{}, v, lds, 0, twM, c, t, lay);
+  __syncthreads();  // the last stage's LDS reads precede the stores below
+#pragma unroll
+  for (int s = 0; s < E; ++s) lds[lay(c, t + out_m<M>(s) * TCOL)] = v[0][s];
+  __syncthreads();
+  // Hermitian half: X[k] = (Z[k] + conj Z[M-k])/2 - i W^k (Z[k] - conj Z[M-k])/2
+  if (!rowok) return;
+  C* dst = out + row * ldo;
+  for (int k = t; k <= M; k += TCOL) {
+    const C zk = lds[lay(c, k == M ? 0 : k)];
+    const C zm = lds[lay(c, k == 0 ? 0 : M - k)];
+    const T ax = zk.x + zm.x, ay = zk.y - zm.y;
+    const T bx = zk.x - zm.x, by = zk.y + zm.y;
+    const C w = twN[k];
+    const T px = w.x * by + w.y * bx;
+    const T py = -(w.x * bx - w.y * by);
+    dst[k] = C{T(0.5) * (ax + px), T(0.5) * (ay + py)};
+  }
+}
+
+// Register-chained C2R rows (see k_fft_rows_c2r): the pre-process runs in LDS,
+// the length-M FFT starts from registers and the real samples go straight
+// from registers to global memory.  nsel == 0 writes whole rows; nsel > 0
+// writes sample (o - off) mod 2M at o < nsel (offm = off mod 2M).  In place
+// when the real rows overlay the complex rows: every workgroup reads its rows
+// before writing them.
+template <typename T, typename Tout, int M>
+__global__ void __launch_bounds__(1024)
+    k_fft_rows_c2r_rc(const cplx<T>* in, int64_t ldi, Tout* out, int64_t ldo, int64_t nrows,
+                      int tr, const cplx<T>* __restrict__ twM, const cplx<T>* __restrict__ twN,
+                      int nsel, int offm) {
+  using C = cplx<T>;
+  constexpr int E = FftCfg<M>::E;
+  constexpr int TCOL = M / E;
+  constexpr int RS = row_pad_len(M);
+  using Rad = typename FftCfg<M>::radices;
+  static_assert(RadixInfo<Rad>::first == E, "first radix must equal E");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  const LayRowPad lay{RS};
+  const int c = threadIdx.x / TCOL, t = threadIdx.x % TCOL;
+  const int64_t row = (int64_t)blockIdx.x * tr + c;
+  const bool rowok = row < nrows;
+  const C* src = in + row * ldi;
+  for (int k = t; k <= M; k += TCOL) {
+    C x{T(0), T(0)};
+    if (rowok) x = src[k];
+    if (k == 0 || k == M) x.y = T(0);
+    lds[lay(c, k)] = x;
+  }
+  __syncthreads();
+  // Z[k] = (X[k] + conj X[M-k]) + i W^-k (X[k] - conj X[M-k]) for k < M, in
+  // pairs (p, M - p) so the update is in place.
+  for (int p = t; p <= M / 2; p += TCOL) {
+    const C xp = lds[lay(c, p)];
+    const C xq = lds[lay(c, M - p)];
+    const C wp = twN[p];
+    const C wq = twN[M - p];
+    {
+      const T ax = xp.x + xq.x, ay = xp.y - xq.y;
+      const T bx = xp.x - xq.x, by = xp.y + xq.y;
+      const T cx = wp.x * bx + wp.y * by, cy = wp.x * by - wp.y * bx;
+      lds[lay(c, p)] = C{ax - cy, ay + cx};
+    }
+    if (p != 0 && p != M - p) {
+      const T ax = xq.x + xp.x, ay = xq.y - xp.y;
+      const T bx = xq.x - xp.x, by = xq.y + xp.y;
+      const T cx = wq.x * bx + wq.y * by, cy = wq.x * by - wq.y * bx;
+      lds[lay(c, M - p)] = C{ax - cy, ay + cx};
+    }
+  }
+  __syncthreads();
+  C v[1][E];
+#pragma unroll
+  for (int s = 0; s < E; ++s) v[0][s] = lds[lay(c, t + s * TCOL)];
+  __syncthreads();  // every first-stage input is read before the first stage stores
+  rstages<T, M, +1, 1, 1, true>(Rad{}, v, lds, 0, twM, c, t, lay);
+  if (!rowok) return;
+  Tout* dst = out + row * ldo;
+  if (nsel == 0) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int e = t + out_m<M>(s) * TCOL;
+      dst[2 * e] = Tout(v[0][s].x);
+      dst[2 * e + 1] = Tout(v[0][s].y);
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int e = t + out_m<M>(s) * TCOL;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        int o = 2 * e + h + offm;
+        if (o >= 2 * M) o -= 2 * M;
+        const Tout val = Tout(h ? v[0][s].y : v[0][s].x);
+        for (; o < nsel; o += 2 * M) dst[o] = val;
+      }
+    }
+  }
+}
+
 // ---- strided column pass ---------------------------------------------------
 // Transforms along L over [A][L][G][B] complex data (G groups of B contiguous
 // columns):

@@ -54,10 +54,45 @@ struct ColView {
   int nout = 0, off = 0;  // nout == 0: all L outputs
 };
 
+template <typename T, int DIR, typename Tin, int TC>
+void launch_cols_rc(int L, const ColView<T, Tin>& v, const cplx<T>* tw, hipStream_t s) {
+  const size_t cs = sizeof(cplx<T>);
+  const int nout = v.nout ? v.nout : L;
+  const int offm = ((v.off % L) + L) % L;
+  const int tilesB = (v.B + TC - 1) / TC;
+  const int64_t nb = (int64_t)v.A * v.G * tilesB;
+  if (nb > INT32_MAX) fail(FC_ERR_INVALID, "transform too large for one launch");
+#define FC_COLS_RC_CASE(Lv)                                                                  \
+  case Lv:                                                                                   \
+    hipLaunchKernelGGL((k_fft_cols_rc<T, Lv, DIR, TC, Tin>), dim3((unsigned)nb),             \
+                       dim3(TC * (Lv / FftCfg<Lv>::E)), (size_t)Lv * TC * cs, s, v.in, v.inA, \
+                       v.inL, v.inG, v.out, v.outA, v.outL, v.outG, v.B, tilesB, v.G, nout,  \
+                       offm, tw);                                                            \
+    break;
+  switch (L) {
+    FC_COLS_RC_CASE(8)
+    FC_COLS_RC_CASE(16)
+    FC_COLS_RC_CASE(32)
+    FC_COLS_RC_CASE(64)
+    FC_COLS_RC_CASE(128)
+    FC_COLS_RC_CASE(256)
+    FC_COLS_RC_CASE(512)
+    default:
+      fail(FC_ERR_RUNTIME, "internal: no register-chained column kernel for this length");
+  }
+#undef FC_COLS_RC_CASE
+  FC_LAUNCH_CHECK();
+}
+
 template <typename T, int DIR, typename Tin = T>
 void launch_cols(int L, const ColView<T, Tin>& v, int tc, const cplx<T>* tw, hipStream_t s) {
   const size_t cs = sizeof(cplx<T>);
   const int nout = v.nout ? v.nout : L;
+  if (rc_len_ok(L) && (size_t)L * tc * cs <= kLdsBudget && tc * (L / 8) <= 1024) {
+    if (tc == 16) return launch_cols_rc<T, DIR, Tin, 16>(L, v, tw, s);
+    if (tc == 8) return launch_cols_rc<T, DIR, Tin, 8>(L, v, tw, s);
+    if (tc == 4) return launch_cols_rc<T, DIR, Tin, 4>(L, v, tw, s);
+  }
 #define FC_COLS_CASE(Lv)                                                                  \
   case Lv: {                                                                              \
     const int thr = tc * (Lv / FftCfg<Lv>::E);                                            \
@@ -121,7 +156,29 @@ void launch_rows_r2c(const FftPlan& p, const Tin* in, int64_t ldi, cplx<T>* out,
   const int tr = p.tile[2];
   const cplx<T>* twM = p.tw2M.as<cplx<T>>();
   const cplx<T>* twN = p.tw2N.as<cplx<T>>();
-  if (p.fast[2]) {
+  if (p.fast[2] && rc_len_ok(M)) {
+    int trc = tr;
+    while (trc > 1 && (size_t)trc * row_pad_len(M) * cs > kLdsBudget) trc /= 2;
+#define FC_R2C_RC_CASE(Mv)                                                                   \
+  case Mv: {                                                                                 \
+    const int thr = trc * (Mv / FftCfg<Mv>::E);                                              \
+    dim3 g((unsigned)((nrows + trc - 1) / trc));                                             \
+    hipLaunchKernelGGL((k_fft_rows_r2c_rc<T, Tin, Mv, Loader>), g, dim3(thr),                \
+                       (size_t)trc * row_pad_len(Mv) * cs, s, in, ldi, out, ldo, nrows, trc,  \
+                       twM, twN, ld);                                                        \
+    break;                                                                                   \
+  }
+    switch (M) {
+      FC_R2C_RC_CASE(8)
+      FC_R2C_RC_CASE(16)
+      FC_R2C_RC_CASE(32)
+      FC_R2C_RC_CASE(64)
+      FC_R2C_RC_CASE(128)
+      FC_R2C_RC_CASE(256)
+      FC_R2C_RC_CASE(512)
+    }
+#undef FC_R2C_RC_CASE
+  } else if (p.fast[2]) {
 #define FC_R2C_CASE(Mv)                                                                     \
   case Mv: {                                                                                \
     const int thr = tr * (Mv / FftCfg<Mv>::E);                                              \
@@ -132,13 +189,6 @@ void launch_rows_r2c(const FftPlan& p, const Tin* in, int64_t ldi, cplx<T>* out,
     break;                                                                                  \
   }
     switch (M) {
-      FC_R2C_CASE(8)
-      FC_R2C_CASE(16)
-      FC_R2C_CASE(32)
-      FC_R2C_CASE(64)
-      FC_R2C_CASE(128)
-      FC_R2C_CASE(256)
-      FC_R2C_CASE(512)
       FC_R2C_CASE(1024)
       FC_R2C_CASE(2048)
       FC_R2C_CASE(24)
@@ -171,7 +221,30 @@ void launch_rows_c2r(const FftPlan& p, const cplx<T>* in, int64_t ldi, Tout* out
   const int tr = p.tile[2];
   const cplx<T>* twM = p.tw2M.as<cplx<T>>();
   const cplx<T>* twN = p.tw2N.as<cplx<T>>();
-  if (p.fast[2]) {
+  if (p.fast[2] && rc_len_ok(M)) {
+    int trc = tr;
+    while (trc > 1 && (size_t)trc * row_pad_len(M) * cs > kLdsBudget) trc /= 2;
+    const int offm = nsel ? ((off % (2 * M)) + 2 * M) % (2 * M) : 0;
+#define FC_C2R_RC_CASE(Mv)                                                                   \
+  case Mv: {                                                                                 \
+    const int thr = trc * (Mv / FftCfg<Mv>::E);                                              \
+    dim3 g((unsigned)((nrows + trc - 1) / trc));                                             \
+    hipLaunchKernelGGL((k_fft_rows_c2r_rc<T, Tout, Mv>), g, dim3(thr),                       \
+                       (size_t)trc * row_pad_len(Mv) * cs, s, in, ldi, out, ldo, nrows, trc,  \
+                       twM, twN, nsel, offm);                                                \
+    break;                                                                                   \
+  }
+    switch (M) {
+      FC_C2R_RC_CASE(8)
+      FC_C2R_RC_CASE(16)
+      FC_C2R_RC_CASE(32)
+      FC_C2R_RC_CASE(64)
+      FC_C2R_RC_CASE(128)
+      FC_C2R_RC_CASE(256)
+      FC_C2R_RC_CASE(512)
+    }
+#undef FC_C2R_RC_CASE
+  } else if (p.fast[2]) {
 #define FC_C2R_CASE(Mv)                                                                    \
   case Mv: {                                                                               \
     const int thr = tr * (Mv / FftCfg<Mv>::E);                                             \
@@ -182,13 +255,6 @@ void launch_rows_c2r(const FftPlan& p, const cplx<T>* in, int64_t ldi, Tout* out
     break;                                                                                 \
   }
     switch (M) {
-      FC_C2R_CASE(8)
-      FC_C2R_CASE(16)
-      FC_C2R_CASE(32)
-      FC_C2R_CASE(64)
-      FC_C2R_CASE(128)
-      FC_C2R_CASE(256)
-      FC_C2R_CASE(512)
       FC_C2R_CASE(1024)
       FC_C2R_CASE(2048)
       FC_C2R_CASE(24)

@@ -49,100 +49,6 @@ __device__ __forceinline__ void put_pk(T* pk, const KSlots& ks, int n0, int n1, 
   }
 }
 
-// ---- register-chained Stockham column FFT ---------------------------------
-// A tile holds TC columns of length L in LDS, element (e, c) at row
-// e ^ ((e >> 3) & 1): rows 8a..8a+7 swap neighbours when bit 3 of the row is
-// set, so the first-stage stores of threads t and t + 1 (rows 8t + r) fall in
-// different LDS banks.
-template <int TC>
-struct LaySwz {
-  __device__ __forceinline__ int operator()(int c, int e) const {
-    return (e ^ ((e >> 3) & 1)) * TC + c;
-  }
-};
-
-template <class R>
-struct RadixInfo;
-template <int R>
-struct RadixInfo<Radices<R>> {
-  static constexpr int first = R, last = R;
-};
-template <int R, int R2, int... Rest>
-struct RadixInfo<Radices<R, R2, Rest...>> {
-  static constexpr int first = R, last = RadixInfo<Radices<R2, Rest...>>::last;
-};
-
-// Thread t of a column keeps E values in registers.  Before the first stage
-// slot s holds element t + s L/E (the first radix equals E).  After the last
-// stage (radix Rl) slot s holds element t + out_m(s) L/E.
-template <int L>
-__host__ __device__ constexpr int out_m(int s) {
-  constexpr int E = FftCfg<L>::E;
-  constexpr int Rl = RadixInfo<typename FftCfg<L>::radices>::last;
-  return s / Rl + (s % Rl) * (E / Rl);
-}
-
-// One Stockham stage on NT tiles in lock-step (one twiddle load serves every
-// tile).  FIRST: the inputs are already in v (no LDS read).  LAST: the
-// outputs stay in v (no LDS write).  A stage that writes LDS after reading it
-// synchronises in between; a stage that writes LDS ends with a barrier.
-template <typename T, int L, int R, int NS, int DIR, int NT, bool FIRST, bool LAST, int TC>
-__device__ __forceinline__ void rstage(cplx<T> (&v)[NT][FftCfg<L>::E], cplx<T>* lds,
-                                       const cplx<T>* __restrict__ tw, int c, int t) {
-  using C = cplx<T>;
-  constexpr int E = FftCfg<L>::E;
-  constexpr int TCOL = L / E;
-  constexpr int NB = E / R;
-  constexpr int LR = L / R;
-  constexpr int TS = L * TC;
-  const LaySwz<TC> lay;
-  if (!FIRST) {
-#pragma unroll
-    for (int q = 0; q < NT; ++q)
-#pragma unroll
-      for (int b = 0; b < NB; ++b)
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[q][b * R + r] = lds[q * TS + lay(c, t + b * TCOL + r * LR)];
-    if (!LAST) __syncthreads();
-  }
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int j = t + b * TCOL;
-    const int k = j % NS;
-    if (NS > 1) {
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        C w = tw[r * k * (L / (NS * R))];
-        if (DIR > 0) w.y = -w.y;
-#pragma unroll
-        for (int q = 0; q < NT; ++q) v[q][b * R + r] = cmul(v[q][b * R + r], w);
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < NT; ++q) Bfly<R, DIR, T>::run(&v[q][b * R]);
-    if (!LAST) {
-      const int base = (j / NS) * NS * R + k;
-#pragma unroll
-      for (int q = 0; q < NT; ++q)
-#pragma unroll
-        for (int r = 0; r < R; ++r) lds[q * TS + lay(c, base + r * NS)] = v[q][b * R + r];
-    }
-  }
-  if (!LAST) __syncthreads();
-}
-
-template <typename T, int L, int DIR, int NT, int NS, bool FIRST, int TC>
-__device__ __forceinline__ void rstages(Radices<>, cplx<T> (&)[NT][FftCfg<L>::E], cplx<T>*,
-                                        const cplx<T>*, int, int) {}
-
-template <typename T, int L, int DIR, int NT, int NS, bool FIRST, int TC, int R, int... Rest>
-__device__ __forceinline__ void rstages(Radices<R, Rest...>, cplx<T> (&v)[NT][FftCfg<L>::E],
-                                        cplx<T>* lds, const cplx<T>* __restrict__ tw, int c,
-                                        int t) {
-  rstage<T, L, R, NS, DIR, NT, FIRST, sizeof...(Rest) == 0, TC>(v, lds, tw, c, t);
-  rstages<T, L, DIR, NT, NS * R, false, TC>(Radices<Rest...>{}, v, lds, tw, c, t);
-}
-
 // ---- the fused kernel -------------------------------------------------------
 // Columns per workgroup: 64 bytes of each row (8 FP32 or 4 FP64 complex).
 template <typename T>
@@ -150,9 +56,7 @@ constexpr int px_tc() {
   return 64 / (int)sizeof(cplx<T>);
 }
 // Lengths the fused kernel takes: power-of-two plans whose first radix is E.
-__host__ __device__ constexpr bool px_len_ok(int L) {
-  return L >= 8 && L <= 512 && (L & (L - 1)) == 0;
-}
+__host__ __device__ constexpr bool px_len_ok(int L) { return rc_len_ok(L); }
 template <typename T, int L>
 constexpr int px_threads() {
   return px_tc<T>() * (L / FftCfg<L>::E);
@@ -200,10 +104,7 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
   C* lds = reinterpret_cast<C*>(smem_raw);
   CI* ldsi = reinterpret_cast<CI*>(smem_raw);
 
-  const int nb = gridDim.x;
-  const int q8 = nb >> 3, r8 = nb & 7;
-  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-  const int tile = (xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8) + slot;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int y = tile / tilesC;
   const int c0 = (tile - y * tilesC) * TC;
   const int ncol = min(TC, n2h - c0);
@@ -226,7 +127,7 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
     for (int s = 0; s < E; ++s)
       g[s] = colok ? B[base + (int64_t)(t + out_m<L>(s) * TCOL) * sL] : C{T(0), T(0)};
   }
-  rstages<T, L, -1, NT, 1, true, TC>(Rad{}, v, lds, tw, c, t);
+  rstages<T, L, -1, NT, 1, true>(Rad{}, v, lds, L * TC, tw, c, t, LaySwz<TC>{});
 
   // Combine: slot s holds element e = t + out_m(s) L/E of both spectra.
   const int oz = ks.oz();
@@ -276,7 +177,7 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
     }
   }
   __syncthreads();  // the forward pass's last LDS reads precede the inverse's stores
-  rstages<TI, L, +1, 1, 1, true, TC>(Rad{}, u, ldsi, twi, c, t);
+  rstages<TI, L, +1, 1, 1, true>(Rad{}, u, ldsi, 0, twi, c, t, LaySwz<TC>{});
   if (colok) {
 #pragma unroll
     for (int s = 0; s < E; ++s) {
