@@ -59,7 +59,6 @@ void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
   start[nbins] = pos;
   if (pos > 0x7fffffffLL) fail(FC_ERR_INVALID, "subgrid too large");
   std::vector<uint32_t> cells(pos > 0 ? pos : 1);
-  std::vector<int> binv(pos > 0 ? pos : 1);
   {
     std::vector<std::thread> th;
     for (unsigned t = 0; t < nt; ++t)
@@ -72,7 +71,6 @@ void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
             for (int k = 0; k < s2; ++k) {
               const int b = bin_of(i, j, k);
               if (b >= 0) {
-                binv[o[b]] = b;
                 cells[o[b]++] = (uint32_t)(((int64_t)i * s1 + j) * s2 + k);
               }
             }
@@ -86,16 +84,32 @@ void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
         bc.tracked_pos = q;
         break;
       }
+  // Chunks of at most kSegChunk entries, never straddling a bin.
+  std::vector<int4> chunks;
+  std::vector<int> first(nbins + 1, 0);
+  for (int b = 0; b < nbins; ++b) {
+    first[b] = (int)chunks.size();
+    for (int64_t q = start[b]; q < start[b + 1]; q += kSegChunk) {
+      const int len = (int)std::min<int64_t>(kSegChunk, start[b + 1] - q);
+      chunks.push_back(make_int4(b, (int)q, len, 0));
+    }
+  }
+  first[nbins] = (int)chunks.size();
   bc.n = pos;
   bc.nbins = nbins;
+  bc.nchunks = (int)chunks.size();
   bc.cells.allocate(sizeof(uint32_t) * cells.size());
-  bc.bins.allocate(sizeof(int) * binv.size());
   bc.counts.allocate(sizeof(int) * nbins);
+  bc.chunks.allocate(sizeof(int4) * std::max<size_t>(chunks.size(), 1));
+  bc.chunk_first.allocate(sizeof(int) * first.size());
   FC_HIP(hipMemcpy(bc.cells.as<void>(), cells.data(), sizeof(uint32_t) * cells.size(),
                    hipMemcpyHostToDevice));
-  FC_HIP(hipMemcpy(bc.bins.as<void>(), binv.data(), sizeof(int) * binv.size(),
-                   hipMemcpyHostToDevice));
   FC_HIP(hipMemcpy(bc.counts.as<void>(), bc.h_counts.data(), sizeof(int) * nbins,
+                   hipMemcpyHostToDevice));
+  if (!chunks.empty())
+    FC_HIP(hipMemcpy(bc.chunks.as<void>(), chunks.data(), sizeof(int4) * chunks.size(),
+                     hipMemcpyHostToDevice));
+  FC_HIP(hipMemcpy(bc.chunk_first.as<void>(), first.data(), sizeof(int) * first.size(),
                    hipMemcpyHostToDevice));
 }
 

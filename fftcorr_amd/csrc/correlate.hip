@@ -47,7 +47,8 @@ struct fc_corr {
   std::vector<double> h_rx, h_kx, h_ky, h_kz;
   fc::DeviceBuffer rx, kx, ky, kz, wx, wy, wz;  // FP64 axes and per-axis window W^2
   fc::BinnedCells kbin, rbin;                   // subgrid cells sorted by bin
-  fc::DeviceBuffer ksrc, kw, rw;                // periodic: pk index + weights per entry
+  fc::DeviceBuffer rw;                          // periodic: r weights per entry
+  double kcoef[3] = {0, 0, 0};                  // periodic k-space Y_l0 coefficients
   // FFT plan + work arrays (complex rows of plan->ldc).
   std::shared_ptr<fc::FftPlan> plan;      // forward precision
   std::shared_ptr<fc::FftPlan> plan_inv;  // inverse precision (FP64 plan for FC_MIXED)
@@ -240,36 +241,63 @@ struct LoadYlm {
   }
 };
 
-// Static weights of the periodic multipole histograms, one per sorted
-// entry and multipole: the k-space weight Y_l0(k) W^-1(k) sqrt(4 pi (2l+1))
-// cell^3 / N (correlator.h:296-320) and the pk index the entry reads; the
-// r-space weight Y_l0(r) sqrt(4 pi (2l+1)) / N^2 (correlator.h:336-355).  A
-// histogram sample is then weight * value, the same product, in the same
-// FP64 operation order, as the reference's make_ylm followed by extraction.
-template <int NV>
-__global__ void k_setup_kweights(const uint32_t* __restrict__ cells, int64_t n, KSlots ks,
-                                 const double* __restrict__ kx, const double* __restrict__ ky,
-                                 const double* __restrict__ kz, const double* __restrict__ wx,
-                                 const double* __restrict__ wy, const double* __restrict__ wz,
-                                 const double* __restrict__ coef, uint32_t* __restrict__ src,
-                                 double* __restrict__ wt) {
+// Histogram entries.  A histogram sample is weight * value with the weight
+// Y_l0 (x) sqrt(4 pi (2l+1)) times the rescaling (and W^-1(k) in k space),
+// formed by the same FP64 operations, in the same order, as the reference's
+// make_ylm followed by extraction (correlator.h:296-320, :336-355).
+//
+// Periodic k space: the value is Re P at the cell's slot of the compact
+// array (the reflected slot for kz < 0, fft_grid.cc:238-245) and the weight
+// is evaluated on the fly from the subgrid axes.
+template <typename T, int NV>
+struct KEntry {
+  const uint32_t* cells;
+  const T* pk;
+  KSlots ks;
+  FastDiv d2, d1;  // division by ks2 and ks1
+  const double *kx, *ky, *kz, *wx, *wy, *wz;
+  double coef[NV];
+  __device__ __forceinline__ void operator()(int64_t pos, double* v) const {
 #pragma clang fp contract(off)
-  const int oz = ks.oz();
-  for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < n;
-       pos += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t cell = cells[pos];
-    const int k = (int)(cell % ks.ks2);
-    const int64_t t = cell / ks.ks2;
-    const int j = (int)(t % ks.ks1), i = (int)(t / ks.ks1);
-    src[pos] = (uint32_t)(k >= oz ? ks.at(i, j, k - oz)
-                                  : ks.at(ks.ks0 - 1 - i, ks.ks1 - 1 - j, oz - k));
+    const uint32_t cell = cells[pos];
+    const uint32_t t = d2.div(cell);
+    const int k = (int)(cell - t * (uint32_t)ks.ks2);
+    const uint32_t iu = d1.div(t);
+    const int j = (int)(t - iu * (uint32_t)ks.ks1), i = (int)iu;
+    const int oz = ks.oz();
+    const int64_t src = k >= oz ? ks.at(i, j, k - oz) : ks.at(ks.ks0 - 1 - i, ks.ks1 - 1 - j, oz - k);
+    const double x = (double)pk[src];
     const double invw = 1.0 / (wx[i] * wy[j] * wz[k]);
 #pragma unroll
-    for (int q = 0; q < NV; ++q)
-      wt[(int64_t)q * n + pos] = ylm_eval(2 * q, 0, kx[i], ky[j], kz[k], coef[q], invw);
+    for (int q = 0; q < NV; ++q) v[q] = ylm_eval(2 * q, 0, kx[i], ky[j], kz[k], coef[q], invw) * x;
   }
-}
+};
 
+// Precomputed weights wt[q][pos] (periodic r space).
+template <typename V, int NV>
+struct WEntry {
+  const V* vals;
+  const uint32_t* cells;
+  const double* wt;
+  int64_t n;
+  __device__ __forceinline__ void operator()(int64_t pos, double* v) const {
+    const double x = (double)vals[cells[pos]];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = wt[(int64_t)q * n + pos] * x;
+  }
+};
+
+// Plain gather (subgrids that already hold weighted values).
+template <typename V>
+struct GEntry {
+  const V* vals;
+  const uint32_t* cells;
+  __device__ __forceinline__ void operator()(int64_t pos, double* v) const {
+    v[0] = (double)vals[cells[pos]];
+  }
+};
+
+// r-space weights, one per sorted entry and multipole (built once).
 template <int NV>
 __global__ void k_setup_rweights(const uint32_t* __restrict__ cells, int64_t n, int rs,
                                  const double* __restrict__ rx, const double* __restrict__ coef,
@@ -740,39 +768,28 @@ void power_and_pruned_inverse(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode
                           (int64_t)rs * rs, rs, o);
 }
 
-// Builds the periodic histogram weights once per correlator.
+// Builds the periodic histogram coefficients and r weights once per correlator.
 template <int NV>
 void ensure_periodic_weights(fc_corr* c) {
-  if (c->kw.bytes()) return;
+  if (c->rw.bytes()) return;
   const uint64_t ncells = (uint64_t)c->shape[0] * c->shape[1] * c->shape[2];
   // correlator.h:294, :310-313 (k_rescale = cell^3 / N) and :336 (1 / N / N).
   const double k_rescale = c->cell * c->cell * c->cell / ncells;
   const double r_rescale = (1.0 / ncells / ncells);
-  double kc[NV], rc[NV];
+  double rc[NV];
   for (int q = 0; q < NV; ++q) {
-    kc[q] = ylm_coeff(2 * q, 0, std::sqrt((4.0 * M_PI) * (2 * (2 * q) + 1)) * k_rescale);
+    c->kcoef[q] = ylm_coeff(2 * q, 0, std::sqrt((4.0 * M_PI) * (2 * (2 * q) + 1)) * k_rescale);
     rc[q] = ylm_coeff(2 * q, 0, std::sqrt((4.0 * M_PI) * (2 * (2 * q) + 1)) * r_rescale);
   }
-  DeviceBuffer coef(sizeof(double) * 2 * NV);
-  std::vector<double> h(kc, kc + NV);
-  h.insert(h.end(), rc, rc + NV);
-  FC_HIP(hipMemcpy(coef.as<void>(), h.data(), sizeof(double) * 2 * NV, hipMemcpyHostToDevice));
-  const int64_t nk = c->kbin.n, nr = c->rbin.n;
-  c->ksrc.allocate(sizeof(uint32_t) * std::max<int64_t>(nk, 1));
-  c->kw.allocate(sizeof(double) * NV * std::max<int64_t>(nk, 1));
+  DeviceBuffer coef(sizeof(double) * NV);
+  FC_HIP(hipMemcpy(coef.as<void>(), rc, sizeof(double) * NV, hipMemcpyHostToDevice));
+  const int64_t nr = c->rbin.n;
   c->rw.allocate(sizeof(double) * NV * std::max<int64_t>(nr, 1));
   hipStream_t s = c->ctx->stream;
-  if (nk > 0)
-    hipLaunchKernelGGL((k_setup_kweights<NV>), dim3(grid1d(nk, 256, 65536)), dim3(256), 0, s,
-                       c->kbin.cells.as<uint32_t>(), nk, kslots(c), c->kx.as<double>(),
-                       c->ky.as<double>(), c->kz.as<double>(), c->wx.as<double>(),
-                       c->wy.as<double>(), c->wz.as<double>(), coef.as<double>(),
-                       c->ksrc.as<uint32_t>(), c->kw.as<double>());
-  FC_LAUNCH_CHECK();
   if (nr > 0)
     hipLaunchKernelGGL((k_setup_rweights<NV>), dim3(grid1d(nr, 256, 65536)), dim3(256), 0, s,
                        c->rbin.cells.as<uint32_t>(), nr, c->rshape, c->rx.as<double>(),
-                       coef.as<double>() + NV, c->rw.as<double>());
+                       coef.as<double>(), c->rw.as<double>());
   FC_LAUNCH_CHECK();
   FC_HIP(hipStreamSynchronize(s));  // coef is freed on return
 }
@@ -783,13 +800,19 @@ void periodic_hists(fc_corr* c) {
   ensure_periodic_weights<NV>(c);
   {
     Stage st(c, FC_STAGE_HIST_K);
-    run_hist_static<NV, T>(c->pk.as<T>(), c->ksrc.as<uint32_t>(), c->kw.as<double>(), c->kbin,
-                           c->hs, c->k_hist.as<double>(), c->k_counts.as<int>(), 0, 0, NV, s);
+    KEntry<T, NV> ke{c->kbin.cells.as<uint32_t>(), c->pk.as<T>(), kslots(c),
+                     FastDiv((uint32_t)c->kshape[2]), FastDiv((uint32_t)c->kshape[1]),
+                     c->kx.as<double>(), c->ky.as<double>(), c->kz.as<double>(),
+                     c->wx.as<double>(), c->wy.as<double>(), c->wz.as<double>(), {}};
+    for (int q = 0; q < NV; ++q) ke.coef[q] = c->kcoef[q];
+    run_hist_seg<NV>(ke, c->kbin, c->hs.part_seg, c->k_hist.as<double>(), c->k_counts.as<int>(),
+                     0, 0, NV, s);
   }
   Stage st(c, FC_STAGE_HIST_R);
-  run_hist_static<NV, TI>(c->xisub.as<TI>(), c->rbin.cells.as<uint32_t>(), c->rw.as<double>(),
-                          c->rbin, c->hs, c->r_hist.as<double>(), c->r_counts.as<int>(), 0, 0,
-                          NV, s, c->rbin.tracked_pos, c->zerolag.as<double>());
+  WEntry<TI, NV> we{c->xisub.as<TI>(), c->rbin.cells.as<uint32_t>(), c->rw.as<double>(),
+                    c->rbin.n};
+  run_hist_seg<NV>(we, c->rbin, c->hs.part_seg, c->r_hist.as<double>(), c->r_counts.as<int>(), 0,
+                   0, NV, s, c->rbin.tracked_pos, c->zerolag.as<double>());
 }
 
 template <typename T, typename TI>
@@ -873,9 +896,9 @@ void hist_subgrids(fc_corr* c, int ell, bool k_side, bool r_side) {
   hipStream_t s = c->ctx->stream;
   if (r_side) {
     Stage st(c, FC_STAGE_HIST_R);
-    run_hist_static<1, double>(c->rsub.as<double>(), c->rbin.cells.as<uint32_t>(), nullptr,
-                               c->rbin, c->hs, c->r_hist.as<double>(), c->r_counts.as<int>(),
-                               ell / 2, ell / 2, ell / 2 + 1, s);
+    run_hist_seg<1>(GEntry<double>{c->rsub.as<double>(), c->rbin.cells.as<uint32_t>()}, c->rbin,
+                    c->hs.part_seg, c->r_hist.as<double>(), c->r_counts.as<int>(), ell / 2,
+                    ell / 2, ell / 2 + 1, s);
     if (ell == 0) {
       const int rs = c->rshape, o = rs / 2;
       const int64_t center = ((int64_t)o * rs + o) * rs + o;
@@ -885,9 +908,9 @@ void hist_subgrids(fc_corr* c, int ell, bool k_side, bool r_side) {
   }
   if (k_side) {
     Stage st(c, FC_STAGE_HIST_K);
-    run_hist_static<1, double>(c->ksub.as<double>(), c->kbin.cells.as<uint32_t>(), nullptr,
-                               c->kbin, c->hs, c->k_hist.as<double>(), c->k_counts.as<int>(),
-                               ell / 2, ell / 2, ell / 2 + 1, s);
+    run_hist_seg<1>(GEntry<double>{c->ksub.as<double>(), c->kbin.cells.as<uint32_t>()}, c->kbin,
+                    c->hs.part_seg, c->k_hist.as<double>(), c->k_counts.as<int>(), ell / 2,
+                    ell / 2, ell / 2 + 1, s);
   }
 }
 

@@ -97,16 +97,22 @@ __global__ void __launch_bounds__(256)
 // cell depends only on the subgrid axes (correlator.h:129-135, :180-186 and
 // histogram_list.h:57), so it is computed once on the host, in FP64 exactly
 // as the reference does, and the cells are stored sorted by bin (raster
-// order within a bin) with each entry's bin beside it.  A wave then reads 64
-// consecutive entries -- almost always one bin -- and the per-bin counts are
+// order within a bin).  Each bin's entries are then one contiguous range,
+// cut into chunks of at most kSegChunk entries: a histogram is a segmented
+// sum, one workgroup per chunk, with no per-entry bin and the per-bin counts
 // known without counting.
+constexpr int kSegThreads = 256;
+constexpr int kSegChunk = kSegThreads * 16;
+
 struct BinnedCells {
   DeviceBuffer cells;        // uint32 linear index (i*s1 + j)*s2 + k, sorted by bin
-  DeviceBuffer bins;         // int32 bin of each entry
   DeviceBuffer counts;       // int32 [nbins]
+  DeviceBuffer chunks;       // int4 {bin, first entry, entries, 0} per chunk
+  DeviceBuffer chunk_first;  // int32 [nbins + 1]: first chunk of each bin
   std::vector<int> h_counts;
   int64_t n = 0;             // entries (cells whose bin is in range)
   int nbins = 0;
+  int nchunks = 0;
   int64_t tracked_pos = -1;  // entry position of the tracked cell (or -1)
 };
 
@@ -115,48 +121,77 @@ void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
                         const std::vector<double>& ay, const std::vector<double>& az,
                         double binsize, int nbins, int64_t track_cell = -1);
 
-// Histogram over bin-sorted entries: value x = vals[src[pos]] (FP64), the NV
-// sample values are wt[q][pos] * x (or x when wt is null).  When pos == zpos
-// the first value is also stored to *zval (the zero-lag cell).
-template <int NV, typename V>
-__global__ void __launch_bounds__(256)
-    k_hist_static(const V* __restrict__ vals, const uint32_t* __restrict__ src,
-                  const int* __restrict__ binof, const double* __restrict__ wt, int64_t n,
-                  int nbins, double* __restrict__ part_hist, int64_t zpos,
-                  double* __restrict__ zval) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int nw = blockDim.x / kWave;
-  double* bins = reinterpret_cast<double*>(smem_raw);
-  int* lcounts = reinterpret_cast<int*>(bins + (size_t)nw * NV * nbins);
-  for (int i = threadIdx.x; i < nw * NV * nbins; i += blockDim.x) bins[i] = 0.0;
-  for (int i = threadIdx.x; i < nbins; i += blockDim.x) lcounts[i] = 0;
-  __syncthreads();
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int64_t lo = (int64_t)blockIdx.x * chunk;
-  const int64_t hi = lo + chunk < n ? lo + chunk : n;
-  double* wb = bins + (size_t)(threadIdx.x / kWave) * NV * nbins;
-  const int64_t rounds = hi > lo ? (hi - lo + blockDim.x - 1) / blockDim.x : 0;
-  for (int64_t r = 0; r < rounds; ++r) {
-    const int64_t pos = lo + r * blockDim.x + threadIdx.x;
-    int bin = -1;
+// Unsigned 32-bit division by a runtime constant d: q = (umulhi(x, m) + x) >> s
+// with the round-up magic m = floor(2^32 (2^s - d) / d) + 1, s = ceil(log2 d)
+// (Granlund and Montgomery, PLDI 1994), the add done in 64 bits.
+struct FastDiv {
+  uint32_t m = 0;
+  int s = 0;
+  FastDiv() = default;
+  explicit FastDiv(uint32_t d) {
+    if (d <= 1) return;
+    int l = 0;
+    while ((1ull << l) < d) ++l;
+    m = (uint32_t)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+    s = l;
+  }
+  __device__ __forceinline__ uint32_t div(uint32_t x) const {
+    const uint64_t t = __umulhi(x, m);
+    return (uint32_t)((t + x) >> s);
+  }
+};
+
+// Segmented histogram over bin-sorted entries: workgroup b sums the NV values
+// g(pos, v) of the entries of chunk b (each thread in entry order, then a
+// fixed tree over lanes and waves) into part[b][NV].  When pos == zpos the
+// first value is also stored to *zval (the zero-lag cell).
+template <int NV, class G>
+__global__ void __launch_bounds__(kSegThreads)
+    k_hist_seg(G g, const int4* __restrict__ chunks, double* __restrict__ part, int64_t zpos,
+               double* __restrict__ zval) {
+  __shared__ double sw[kSegThreads / kWave][NV];
+  const int4 ch = chunks[blockIdx.x];
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+  for (int i = threadIdx.x; i < ch.z; i += kSegThreads) {
+    const int64_t pos = (int64_t)ch.y + i;
     double v[NV];
+    g(pos, v);
 #pragma unroll
-    for (int q = 0; q < NV; ++q) v[q] = 0.0;
-    if (pos < hi) {
-      bin = binof[pos];
-      const double x = (double)vals[src[pos]];
+    for (int q = 0; q < NV; ++q) acc[q] += v[q];
+    if (pos == zpos) *zval = v[0];
+  }
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
 #pragma unroll
-      for (int q = 0; q < NV; ++q) v[q] = wt ? wt[(int64_t)q * n + pos] * x : x;
-      if (pos == zpos) *zval = v[0];
-    }
-    wave_hist_add<NV>(bin, v, wb, lcounts, nbins);
+  for (int q = 0; q < NV; ++q) {
+    const double s = wave_sum(acc[q]);
+    if (lane == 0) sw[w][q] = s;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < NV * nbins; i += blockDim.x) {
+  if (threadIdx.x < NV) {
     double s = 0.0;
-    for (int w = 0; w < nw; ++w) s += bins[(size_t)w * NV * nbins + i];
-    part_hist[(size_t)blockIdx.x * NV * nbins + i] = s;
+    for (int k = 0; k < kSegThreads / kWave; ++k) s += sw[k][threadIdx.x];
+    part[(size_t)blockIdx.x * NV + threadIdx.x] = s;
   }
+}
+
+// hist[row0 + q][bin] += the chunk partials of the bin in chunk order;
+// counts rows [crow0, crow1) += counts of the bin.
+template <int NV>
+__global__ void __launch_bounds__(256)
+    k_hist_seg_final(const double* __restrict__ part, const int* __restrict__ chunk_first,
+                     const int* __restrict__ bin_counts, int nbins, double* __restrict__ hist,
+                     int* __restrict__ counts, int row0, int crow0, int crow1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < NV * nbins) {
+    const int q = i / nbins, bin = i - q * nbins;
+    double s = 0.0;
+    for (int c = chunk_first[bin]; c < chunk_first[bin + 1]; ++c) s += part[(size_t)c * NV + q];
+    hist[(size_t)(row0 + q) * nbins + bin] += s;
+  }
+  if (i < nbins)
+    for (int r = crow0; r < crow1; ++r) counts[(size_t)r * nbins + i] += bin_counts[i];
 }
 
 // One block per (multipole, bin): hist[row0 + q][bin] += the block partials
@@ -197,31 +232,28 @@ __global__ void __launch_bounds__(256)
 
 // Scratch for one histogram pass.
 struct HistScratch {
-  DeviceBuffer part_hist, part_counts;
+  DeviceBuffer part_hist, part_counts, part_seg;
   void reserve(int nv, int nbins) {
     part_hist.reserve(sizeof(double) * (size_t)kHistBlocks * nv * nbins);
     part_counts.reserve(sizeof(int) * (size_t)kHistBlocks * nbins);
   }
 };
 
-template <int NV, typename V>
-void run_hist_static(const V* vals, const uint32_t* src, const double* wt, const BinnedCells& bc,
-                     HistScratch& hs, double* hist, int* counts, int row0, int crow0, int crow1,
-                     hipStream_t s, int64_t zpos = -1, double* zval = nullptr) {
+template <int NV, class G>
+void run_hist_seg(G g, const BinnedCells& bc, DeviceBuffer& part, double* hist, int* counts,
+                  int row0, int crow0, int crow1, hipStream_t s, int64_t zpos = -1,
+                  double* zval = nullptr) {
   const int nbins = bc.nbins;
-  const int thr = hist_threads(NV, nbins);
-  if (thr == 0) fail(FC_ERR_INVALID, "too many histogram bins (" + std::to_string(nbins) + ")");
-  hs.reserve(NV, nbins);
-  int nb = (int)std::min<int64_t>(kHistBlocks, (bc.n + thr - 1) / thr);
-  if (nb < 1) nb = 1;
-  hipLaunchKernelGGL((k_hist_static<NV, V>), dim3(nb), dim3(thr), hist_lds_bytes(NV, nbins, thr),
-                     s, vals, src, bc.bins.as<int>(), wt, bc.n, nbins, hs.part_hist.as<double>(),
-                     zpos, zval);
-  FC_LAUNCH_CHECK();
+  if (bc.nchunks > 0) {
+    part.reserve(sizeof(double) * (size_t)bc.nchunks * NV);
+    hipLaunchKernelGGL((k_hist_seg<NV, G>), dim3(bc.nchunks), dim3(kSegThreads), 0, s, g,
+                       bc.chunks.as<int4>(), part.as<double>(), zpos, zval);
+    FC_LAUNCH_CHECK();
+  }
   const int tot = NV * nbins > nbins ? NV * nbins : nbins;
-  hipLaunchKernelGGL((k_hist_final<NV>), dim3(tot), dim3(256), 0, s, hs.part_hist.as<double>(),
-                     (const int*)nullptr, bc.counts.as<int>(), nb, nbins, hist, counts, row0,
-                     crow0, crow1);
+  hipLaunchKernelGGL((k_hist_seg_final<NV>), dim3((tot + 255) / 256), dim3(256), 0, s,
+                     part.as<double>(), bc.chunk_first.as<int>(), bc.counts.as<int>(), nbins,
+                     hist, counts, row0, crow0, crow1);
   FC_LAUNCH_CHECK();
 }
 
