@@ -234,8 +234,10 @@ __global__ void __launch_bounds__(1024)
 struct LoadYlm {
   int ell, m, n1;
   double c, x0, y0, z0;  // cell centres relative to the observer: x0 + i etc.
+  int i0 = 0;            // x slab of row 0
+  void shift_x(int dx) { i0 += dx; }
   __device__ __forceinline__ double operator()(double v, int64_t row, int k) const {
-    const int i = (int)(row / n1), j = (int)(row % n1);
+    const int i = i0 + (int)(row / n1), j = (int)(row % n1);
     return ylm_eval(ell, m, x0 + (double)i * 1.0, y0 + (double)j * 1.0, z0 + (double)k * 1.0,
                     c, v);
   }
@@ -549,19 +551,32 @@ void check_grid_shape(fc_corr* c, fc_grid* g) {
 }
 
 // z and y forward passes of a device grid into dst (complex rows of ldc).
+// The grid is processed in chunks of x slabs (plan zy_chunk) so that each
+// chunk's y pass reads the z pass's output from the last-level cache.
 template <typename T, class Loader>
 void forward_zy(fc_corr* c, fc_grid* g, cplx<T>* dst, Loader ld) {
   const FftPlan& p = *c->plan;
   hipStream_t s = c->ctx->stream;
-  {
-    Stage st(c, FC_STAGE_FFT_Z_FWD);
-    if (g->prec == FC_FP64)
-      launch_rows_r2c<T, double>(p, g->data.as<double>(), p.n[2], dst, p.ldc, ld, s);
-    else
-      launch_rows_r2c<T, float>(p, g->data.as<float>(), p.n[2], dst, p.ldc, ld, s);
+  const int X = p.zy_chunk > 0 ? p.zy_chunk : p.n[0];
+  for (int x0 = 0; x0 < p.n[0]; x0 += X) {
+    const int nx = std::min(X, p.n[0] - x0);
+    const int64_t rows = (int64_t)nx * p.n[1];
+    cplx<T>* d = dst + (int64_t)x0 * p.n[1] * p.ldc;
+    Loader lc = ld;
+    lc.shift_x(x0);
+    {
+      Stage st(c, FC_STAGE_FFT_Z_FWD);
+      const int64_t in0 = (int64_t)x0 * p.n[1] * p.n[2];
+      if (g->prec == FC_FP64)
+        launch_rows_r2c<T, double>(p, g->data.as<double>() + in0, p.n[2], d, p.ldc, lc, s, rows);
+      else
+        launch_rows_r2c<T, float>(p, g->data.as<float>() + in0, p.n[2], d, p.ldc, lc, s, rows);
+    }
+    Stage st(c, FC_STAGE_FFT_Y_FWD);
+    ColView<T> v = y_view<T>(p, d);
+    v.A = nx;
+    launch_cols<T, -1>(p.n[1], v, p.tile[1], p.tw1.as<cplx<T>>(), s);
   }
-  Stage st(c, FC_STAGE_FFT_Y_FWD);
-  launch_cols<T, -1>(p.n[1], y_view<T>(p, dst), p.tile[1], p.tw1.as<cplx<T>>(), s);
 }
 
 template <typename T>

@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <fstream>
 #include <sstream>
+#include <algorithm>
 
 #include "fft_plan.h"
 
@@ -33,8 +34,9 @@ struct PlanKey {
 
 std::mutex g_plan_mu;
 std::map<PlanKey, std::shared_ptr<FftPlan>> g_plans;
-// Tile overrides from an imported wisdom file: (axis length, precision, kind) -> tile.
-// kind 0 = column pass, 1 = row pass.
+// Tile overrides (imported wisdom or the autotuner): (axis length, precision,
+// kind) -> tile.  kind 0 = column pass (columns per workgroup), 1 = row pass
+// (rows per workgroup), 2 = z+y chunk size in KiB (length 0).
 std::map<std::tuple<int, int, int>, int> g_tile_wisdom;
 
 template <typename T>
@@ -49,6 +51,8 @@ void upload_twiddles(DeviceBuffer& buf, int L) {
   FC_HIP(hipMemcpy(buf.as<void>(), h.data(), sizeof(cplx<T>) * (size_t)L,
                    hipMemcpyHostToDevice));
 }
+
+}  // namespace
 
 int default_col_tile(int L, size_t cs) {
   // Runs of 128 bytes per global access: 16 FP32 / 8 FP64 complex columns.
@@ -67,20 +71,10 @@ int default_row_tile(int M, size_t cs) {
   return tr;
 }
 
-}  // namespace
 
-std::shared_ptr<FftPlan> get_fft_plan(int device, const int shape[3], int prec) {
-  for (int i = 0; i < 3; ++i) {
-    if (shape[i] <= 0) fail(FC_ERR_INVALID, "FFT shape must be positive");
-    if (shape[i] > kMaxFftLen)
-      fail(FC_ERR_INVALID, "FFT length " + std::to_string(shape[i]) + " exceeds " +
-                               std::to_string(kMaxFftLen));
-  }
-  if (prec != FC_FP64 && prec != FC_FP32) fail(FC_ERR_INVALID, "unknown precision");
-  PlanKey key{device, shape[0], shape[1], shape[2], prec};
-  std::lock_guard<std::mutex> lk(g_plan_mu);
-  auto it = g_plans.find(key);
-  if (it != g_plans.end()) return it->second;
+namespace {
+// Builds a plan with the current wisdom (caller holds g_plan_mu).
+std::shared_ptr<FftPlan> build_plan(int device, const int shape[3], int prec) {
   auto p = std::make_shared<FftPlan>();
   p->device = device;
   for (int i = 0; i < 3; ++i) p->n[i] = shape[i];
@@ -100,12 +94,19 @@ std::shared_ptr<FftPlan> get_fft_plan(int device, const int shape[3], int prec) 
       fail(FC_ERR_INVALID, "FFT length " + std::to_string(shape[i]) +
                                " needs a 2^a or 3*2^a size at this precision");
   auto tile_for = [&](int L, int kind, int dflt) {
-    auto w = g_tile_wisdom.find(std::make_tuple(L, prec, kind));
+    auto w = g_tile_wisdom.find(std::make_tuple(L, kind == 2 ? 0 : prec, kind));
     return w != g_tile_wisdom.end() ? w->second : dflt;
   };
   p->tile[0] = tile_for(shape[0], 0, default_col_tile(shape[0], cs));
   p->tile[1] = tile_for(shape[1], 0, default_col_tile(shape[1], cs));
   p->tile[2] = p->fast[2] ? tile_for(shape[2] / 2, 1, default_row_tile(shape[2] / 2, cs)) : 1;
+  p->zy_chunk_kib = tile_for(0, 2, kDefaultZyChunkKiB);
+  {
+    const size_t slab = (size_t)shape[1] * p->ldc * cs;
+    const size_t want = (size_t)p->zy_chunk_kib << 10;
+    p->zy_chunk = want ? (int)std::max<size_t>(1, want / slab) : 0;
+    if (p->zy_chunk >= shape[0]) p->zy_chunk = 0;
+  }
   DeviceGuard g(device);
   if (prec == FC_FP64) {
     upload_twiddles<double>(p->tw0, shape[0]);
@@ -118,13 +119,40 @@ std::shared_ptr<FftPlan> get_fft_plan(int device, const int shape[3], int prec) 
     upload_twiddles<float>(p->tw2N, shape[2]);
     if (p->fast[2]) upload_twiddles<float>(p->tw2M, shape[2] / 2);
   }
+  return p;
+}
+
+void check_plan_args(const int shape[3], int prec) {
+  for (int i = 0; i < 3; ++i) {
+    if (shape[i] <= 0) fail(FC_ERR_INVALID, "FFT shape must be positive");
+    if (shape[i] > kMaxFftLen)
+      fail(FC_ERR_INVALID, "FFT length " + std::to_string(shape[i]) + " exceeds " +
+                               std::to_string(kMaxFftLen));
+  }
+  if (prec != FC_FP64 && prec != FC_FP32) fail(FC_ERR_INVALID, "unknown precision");
+}
+}  // namespace
+
+std::shared_ptr<FftPlan> get_fft_plan(int device, const int shape[3], int prec) {
+  check_plan_args(shape, prec);
+  PlanKey key{device, shape[0], shape[1], shape[2], prec};
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  auto it = g_plans.find(key);
+  if (it != g_plans.end()) return it->second;
+  auto p = build_plan(device, shape, prec);
   g_plans[key] = p;
   return p;
 }
 
+std::shared_ptr<FftPlan> make_private_plan(int device, const int shape[3], int prec) {
+  check_plan_args(shape, prec);
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  return build_plan(device, shape, prec);
+}
+
 void set_tile_wisdom(int L, int prec, int kind, int tile) {
   std::lock_guard<std::mutex> lk(g_plan_mu);
-  g_tile_wisdom[std::make_tuple(L, prec, kind)] = tile;
+  g_tile_wisdom[std::make_tuple(L, kind == 2 ? 0 : prec, kind)] = tile;
   g_plans.clear();
 }
 
@@ -227,6 +255,7 @@ int fc_plan_cache_export(const char* path) {
       for (int ax = 0; ax < 2; ++ax)
         all.emplace(std::make_tuple(p.n[ax], p.prec, 0), p.tile[ax]);
       if (p.fast[2]) all.emplace(std::make_tuple(p.n[2] / 2, p.prec, 1), p.tile[2]);
+      all.emplace(std::make_tuple(0, 0, 2), p.zy_chunk_kib);
     }
     for (auto& kv : all)
       f << std::get<0>(kv.first) << " " << std::get<1>(kv.first) << " "
@@ -247,8 +276,10 @@ int fc_plan_cache_import(const char* path) {
     int L, prec, kind, tile;
     std::vector<std::array<int, 4>> rows;
     while (f >> L >> prec >> kind >> tile) {
-      if (L <= 0 || tile <= 0 || (prec != FC_FP64 && prec != FC_FP32) || kind < 0 || kind > 1)
-        fc::fail(FC_ERR_RUNTIME, std::string("corrupt plan cache file: ") + path);
+      const bool ok = kind == 2 ? (L == 0 && prec == 0 && tile >= 0)
+                                : (L > 0 && tile > 0 && (prec == FC_FP64 || prec == FC_FP32) &&
+                                   (kind == 0 || kind == 1));
+      if (!ok) fc::fail(FC_ERR_RUNTIME, std::string("corrupt plan cache file: ") + path);
       rows.push_back({L, prec, kind, tile});
     }
     for (auto& r : rows) fc::set_tile_wisdom(r[0], r[1], r[2], r[3]);

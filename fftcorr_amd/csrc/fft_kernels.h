@@ -654,6 +654,7 @@ __global__ void __launch_bounds__(256)
 // type Tin) and [nrows][ldo] complex outputs.
 struct LoadCopy {
   __device__ __forceinline__ double operator()(double v, int64_t, int) const { return v; }
+  void shift_x(int) {}  // rows restart at a later x slab: nothing to adjust
 };
 
 // block = tr * (M / E) threads, handling rows blockIdx.x*tr ... ; LDS tr*(M+1).

@@ -11,6 +11,8 @@
 namespace fc {
 
 constexpr int kMaxFftLen = 4096;
+// Default z+y chunk (KiB of complex output per chunk; 0 = no chunking).
+constexpr int kDefaultZyChunkKiB = 0;
 constexpr size_t kLdsBudget = 65536;  // bytes of LDS per workgroup we allow
 
 // Tile shape chosen for one transform axis (autotuned; see fft.hip).
@@ -28,6 +30,10 @@ struct FftPlan {
   int prec = FC_FP64;
   bool fast[3] = {false, false, false};  // axis 2: the half-length row FFT
   int tile[3] = {0, 0, 0};
+  // x slabs per z+y chunk: the y pass of a chunk reads the z pass's output
+  // while it is still in the last-level cache (0: the whole grid at once).
+  int zy_chunk = 0;
+  int zy_chunk_kib = 0;
   // Twiddle tables exp(-2 pi i m / L) in the plan's precision.
   DeviceBuffer tw0, tw1;      // lengths n0, n1
   DeviceBuffer tw2M, tw2N;    // lengths n2/2 (fast rows) and n2 (post-process / DFT)
@@ -41,6 +47,11 @@ int fft_len_e(int L);
 
 // Plan cache keyed by (device, shape, precision).
 std::shared_ptr<FftPlan> get_fft_plan(int device, const int shape[3], int prec);
+// An uncached plan built with the current wisdom (the autotuner's scratch plan).
+std::shared_ptr<FftPlan> make_private_plan(int device, const int shape[3], int prec);
+void set_tile_wisdom(int L, int prec, int kind, int tile);
+int default_col_tile(int L, size_t cs);
+int default_row_tile(int M, size_t cs);
 
 // ---- launchers -------------------------------------------------------------
 // Column view [A][L][G][B] for k_fft_cols / k_dft_cols (see fft_kernels.h).
@@ -149,9 +160,9 @@ ColView<T> x_view(const FftPlan& p, cplx<T>* d) {
 
 template <typename T, typename Tin, class Loader>
 void launch_rows_r2c(const FftPlan& p, const Tin* in, int64_t ldi, cplx<T>* out, int64_t ldo,
-                     Loader ld, hipStream_t s) {
+                     Loader ld, hipStream_t s, int64_t nrows = -1) {
   const size_t cs = sizeof(cplx<T>);
-  const int64_t nrows = p.nrows();
+  if (nrows < 0) nrows = p.nrows();
   const int M = p.n[2] / 2;
   const int tr = p.tile[2];
   const cplx<T>* twM = p.tw2M.as<cplx<T>>();

@@ -188,6 +188,17 @@ int fc_fft_c2r(fc_ctx* ctx, const int shape[3], int precision,
 int fc_plan_cache_export(const char* path);
 int fc_plan_cache_import(const char* path);
 int fc_plan_cache_forget(void);
+/* Autotuner (the GPU analogue of FFTW_MEASURE planning, fft_grid.cc:69-115):
+ * times the candidate tiles of every pass of a [shape] plan at `precision`
+ * (FC_MIXED tunes the FP32 forward plan) on ctx's device, reps timed runs
+ * each, and records the fastest in the plan cache (exported by
+ * fc_plan_cache_export).  best_ms (may be NULL) receives the best times of
+ * the z pass, y pass, x pass and chunked z+y sequence.  Plans created
+ * afterwards use the tuned tiles; existing correlators keep theirs. */
+int fc_autotune(fc_ctx* ctx, const int shape[3], int precision, int reps, double best_ms[4]);
+/* Tiles a new plan of this shape would use: {x columns, y columns, z rows,
+ * z+y chunk KiB (0: unchunked)}. */
+int fc_plan_tiles(int device, const int shape[3], int precision, int tiles[4]);
 
 #ifdef __cplusplus
 }
