@@ -699,10 +699,19 @@ void launch_periodic_x(fc_corr* c, cplx<T>* a, const cplx<T>* b) {
   FC_LAUNCH_CHECK();
 }
 
+// The register-chained kernel addresses its arrays with 32-bit byte offsets.
+bool px_fits(const fc_corr* c) {
+  const FftPlan& p = *c->plan;
+  const FftPlan& pi = *c->plan_inv;
+  const uint64_t lim = 0xffffffffull;
+  const uint64_t q1 = (uint64_t)c->rshape * pi.n[1] * pi.ldc * pi.csize();
+  const uint64_t pk = (uint64_t)c->kshape[0] * c->kshape[1] * (c->kshape[2] / 2 + 1) * 8;
+  return px_len_ok(p.n[0]) && p.complex_bytes() <= lim && q1 <= lim && pk <= lim;
+}
+
 template <typename T, typename TI, int MODE>
 void fused_x(fc_corr* c, cplx<T>* w, const cplx<T>* b) {
-  const int L = c->plan->n[0];
-  if (px_len_ok(L))
+  if (px_fits(c))
     launch_periodic_x2<T, TI, MODE>(c, w, b);
   else if constexpr (std::is_same<T, TI>::value)
     launch_periodic_x<T, MODE>(c, w, b);
@@ -751,7 +760,8 @@ void power_and_pruned_inverse(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode
   const FftPlan& pi = *c->plan_inv;
   const int64_t ldq = pi.ldc;
   const int rs = c->rshape, o = rs / 2;
-  const bool fused = px_len_ok(p.n[0]) || (p.fast[0] && std::is_same<T, TI>::value);
+  const bool fused = px_fits(c) || (p.fast[0] && !px_len_ok(p.n[0]) &&
+                                    std::is_same<T, TI>::value);
   if (fused) {
     Stage st(c, FC_STAGE_COMBINE);
     if (mode == kModeInterlaced)

@@ -39,17 +39,48 @@ std::map<PlanKey, std::shared_ptr<FftPlan>> g_plans;
 // (rows per workgroup), 2 = z+y chunk size in KiB (length 0).
 std::map<std::tuple<int, int, int>, int> g_tile_wisdom;
 
+int stage_len(int L) {
+  switch (L) {
+    case 8: return stage_twiddle_len<8>();
+    case 16: return stage_twiddle_len<16>();
+    case 32: return stage_twiddle_len<32>();
+    case 64: return stage_twiddle_len<64>();
+    case 128: return stage_twiddle_len<128>();
+    case 256: return stage_twiddle_len<256>();
+    case 512: return stage_twiddle_len<512>();
+    default: return 0;
+  }
+}
+
+template <int L, class Put>
+void stage_entries(Put put) {
+  stage_twiddles_<L>(typename FftCfg<L>::radices{}, put);
+}
+
+// Plain table exp(-2 pi i m / L), m < L, followed (register-chained lengths)
+// by the per-stage table of the same values regrouped per thread.
 template <typename T>
 void upload_twiddles(DeviceBuffer& buf, int L) {
-  std::vector<cplx<T>> h(L);
+  const int ns = rc_len_ok(L) ? stage_len(L) : 0;
+  std::vector<cplx<T>> h(L + ns);
   for (int m = 0; m < L; ++m) {
     long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)m / L;
     h[m].x = (T)cosl(a);
     h[m].y = (T)(-sinl(a));
   }
-  buf.allocate(sizeof(cplx<T>) * (size_t)L);
-  FC_HIP(hipMemcpy(buf.as<void>(), h.data(), sizeof(cplx<T>) * (size_t)L,
-                   hipMemcpyHostToDevice));
+  auto put = [&](int idx, int m) { h[L + idx] = h[m]; };
+  switch (ns ? L : 0) {
+    case 8: stage_entries<8>(put); break;
+    case 16: stage_entries<16>(put); break;
+    case 32: stage_entries<32>(put); break;
+    case 64: stage_entries<64>(put); break;
+    case 128: stage_entries<128>(put); break;
+    case 256: stage_entries<256>(put); break;
+    case 512: stage_entries<512>(put); break;
+    default: break;
+  }
+  buf.allocate(sizeof(cplx<T>) * h.size());
+  FC_HIP(hipMemcpy(buf.as<void>(), h.data(), sizeof(cplx<T>) * h.size(), hipMemcpyHostToDevice));
 }
 
 }  // namespace

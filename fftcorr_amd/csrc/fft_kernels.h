@@ -299,9 +299,15 @@ __host__ __device__ constexpr int row_pad_len(int M) { return M + M / 8 + 1; }
 // stage that reads and then writes LDS synchronises in between; a stage that
 // writes LDS ends with a barrier.  A FIRST stage writes without a barrier
 // before it: the caller makes sure no thread still reads the LDS it writes.
-template <typename T, int L, int R, int NS, int DIR, int NT, bool FIRST, bool LAST, class Lay>
+//
+// Twiddles come from the per-stage table that follows the L plain twiddles
+// of the axis (see stage_twiddle_len): the stage with prefix NS owns NS * R
+// entries at OFF, entry k * R + r = exp(-2 pi i r k / (NS R)), so a thread's
+// R - 1 twiddles of one stage are contiguous.
+template <typename T, int L, int R, int NS, int OFF, int DIR, int NT, bool FIRST, bool LAST,
+          class Lay>
 __device__ __forceinline__ void rstage(cplx<T> (&v)[NT][FftCfg<L>::E], cplx<T>* lds, int ts,
-                                       const cplx<T>* __restrict__ tw, int c, int t, Lay lay) {
+                                       const cplx<T>* __restrict__ tws, int c, int t, Lay lay) {
   using C = cplx<T>;
   constexpr int E = FftCfg<L>::E;
   constexpr int TCOL = L / E;
@@ -321,9 +327,10 @@ __device__ __forceinline__ void rstage(cplx<T> (&v)[NT][FftCfg<L>::E], cplx<T>* 
     const int j = t + b * TCOL;
     const int k = j % NS;
     if (NS > 1) {
+      const C* wk = tws + OFF + k * R;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        C w = tw[r * k * (L / (NS * R))];
+        C w = wk[r];
         if (DIR > 0) w.y = -w.y;
 #pragma unroll
         for (int q = 0; q < NT; ++q) v[q][b * R + r] = cmul(v[q][b * R + r], w);
@@ -342,16 +349,48 @@ __device__ __forceinline__ void rstage(cplx<T> (&v)[NT][FftCfg<L>::E], cplx<T>* 
   if (!LAST) __syncthreads();
 }
 
-template <typename T, int L, int DIR, int NT, int NS, bool FIRST, class Lay>
-__device__ __forceinline__ void rstages(Radices<>, cplx<T> (&)[NT][FftCfg<L>::E], cplx<T>*, int,
-                                        const cplx<T>*, int, int, Lay) {}
+template <typename T, int L, int DIR, int NT, int NS, int OFF, bool FIRST, class Lay>
+__device__ __forceinline__ void rstages_(Radices<>, cplx<T> (&)[NT][FftCfg<L>::E], cplx<T>*, int,
+                                         const cplx<T>*, int, int, Lay) {}
 
-template <typename T, int L, int DIR, int NT, int NS, bool FIRST, class Lay, int R, int... Rest>
-__device__ __forceinline__ void rstages(Radices<R, Rest...>, cplx<T> (&v)[NT][FftCfg<L>::E],
-                                        cplx<T>* lds, int ts, const cplx<T>* __restrict__ tw,
-                                        int c, int t, Lay lay) {
-  rstage<T, L, R, NS, DIR, NT, FIRST, sizeof...(Rest) == 0>(v, lds, ts, tw, c, t, lay);
-  rstages<T, L, DIR, NT, NS * R, false>(Radices<Rest...>{}, v, lds, ts, tw, c, t, lay);
+template <typename T, int L, int DIR, int NT, int NS, int OFF, bool FIRST, class Lay, int R,
+          int... Rest>
+__device__ __forceinline__ void rstages_(Radices<R, Rest...>, cplx<T> (&v)[NT][FftCfg<L>::E],
+                                         cplx<T>* lds, int ts, const cplx<T>* __restrict__ tws,
+                                         int c, int t, Lay lay) {
+  rstage<T, L, R, NS, OFF, DIR, NT, FIRST, sizeof...(Rest) == 0>(v, lds, ts, tws, c, t, lay);
+  rstages_<T, L, DIR, NT, NS * R, (NS > 1 ? OFF + NS * R : OFF), false>(Radices<Rest...>{}, v,
+                                                                        lds, ts, tws, c, t, lay);
+}
+
+// All stages; tw is the axis's twiddle table (plain part then stage part).
+template <typename T, int L, int DIR, int NT, int NS, bool FIRST, class Lay, class Rad>
+__device__ __forceinline__ void rstages(Rad, cplx<T> (&v)[NT][FftCfg<L>::E], cplx<T>* lds,
+                                        int ts, const cplx<T>* __restrict__ tw, int c, int t,
+                                        Lay lay) {
+  rstages_<T, L, DIR, NT, NS, 0, FIRST>(Rad{}, v, lds, ts, tw + L, c, t, lay);
+}
+
+// Host: length of the per-stage table of an L-point register-chained plan,
+// and its entries as indices m into the plain table exp(-2 pi i m / L).
+template <int L, int NS = 1, int R, int... Rest>
+constexpr int stage_twiddle_len_(Radices<R, Rest...>) {
+  if constexpr (sizeof...(Rest) == 0)
+    return NS > 1 ? NS * R : 0;
+  else
+    return (NS > 1 ? NS * R : 0) + stage_twiddle_len_<L, NS * R>(Radices<Rest...>{});
+}
+template <int L>
+constexpr int stage_twiddle_len() {
+  return stage_twiddle_len_<L>(typename FftCfg<L>::radices{});
+}
+template <int L, int NS = 1, int OFF = 0, int R, int... Rest, class Put>
+void stage_twiddles_(Radices<R, Rest...>, Put put) {
+  if (NS > 1)
+    for (int k = 0; k < NS; ++k)
+      for (int r = 0; r < R; ++r) put(OFF + k * R + r, r * k * (L / (NS * R)));
+  if constexpr (sizeof...(Rest) > 0)
+    stage_twiddles_<L, NS * R, (NS > 1 ? OFF + NS * R : OFF)>(Radices<Rest...>{}, put);
 }
 
 // XCD-aware block order: the hardware deals consecutive workgroups to the 8

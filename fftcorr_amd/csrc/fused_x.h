@@ -111,21 +111,27 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
   const int c = threadIdx.x % TC, t = threadIdx.x / TC;
   const int cg = c0 + c;
   const bool colok = c < ncol;
-  const int64_t sL = (int64_t)n1 * ldc;
-  const int64_t base = (int64_t)y * ldc + cg;
+  // Byte offsets stay below 4 GB (checked by the launcher): 32-bit address
+  // arithmetic on a uniform base.
+  const uint32_t sL = (uint32_t)n1 * (uint32_t)ldc * (uint32_t)sizeof(C);
+  const uint32_t base = ((uint32_t)y * (uint32_t)ldc + (uint32_t)cg) * (uint32_t)sizeof(C);
+  const char* Ab = reinterpret_cast<const char*>(A);
+  const char* Bb = reinterpret_cast<const char*>(B);
 
   C v[NT][E];
   C g[MODE == kModeCross ? E : 1];
 #pragma unroll
   for (int s = 0; s < E; ++s) {
-    const int64_t o = base + (int64_t)(t + s * TCOL) * sL;
-    v[0][s] = colok ? A[o] : C{T(0), T(0)};
-    if (NT == 2) v[NT - 1][s] = colok ? B[o] : C{T(0), T(0)};
+    const uint32_t o = base + (uint32_t)(t + s * TCOL) * sL;
+    v[0][s] = colok ? *reinterpret_cast<const C*>(Ab + o) : C{T(0), T(0)};
+    if (NT == 2) v[NT - 1][s] = colok ? *reinterpret_cast<const C*>(Bb + o) : C{T(0), T(0)};
   }
   if (MODE == kModeCross) {
 #pragma unroll
-    for (int s = 0; s < E; ++s)
-      g[s] = colok ? B[base + (int64_t)(t + out_m<L>(s) * TCOL) * sL] : C{T(0), T(0)};
+    for (int s = 0; s < E; ++s) {
+      const uint32_t o = base + (uint32_t)(t + out_m<L>(s) * TCOL) * sL;
+      g[s] = colok ? *reinterpret_cast<const C*>(Bb + o) : C{T(0), T(0)};
+    }
   }
   rstages<T, L, -1, NT, 1, true>(Rad{}, v, lds, L * TC, tw, c, t, LaySwz<TC>{});
 
@@ -135,8 +141,8 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
   const int ox = ks.ks0 / 2, oy = ks.ks1 / 2;
   int sy0 = slot_a(y, oy), sy1 = slot_b(y, n1, oy);
   if (sy1 == sy0) sy1 = -1;
-  const int64_t pstride = (int64_t)ks.ks1 * (oz + 1);
-  const int64_t prow0 = (int64_t)sy0 * (oz + 1) + cg, prow1 = (int64_t)sy1 * (oz + 1) + cg;
+  const uint32_t pstride = (uint32_t)ks.ks1 * (uint32_t)(oz + 1);
+  const uint32_t prow0 = (uint32_t)(sy0 * (oz + 1) + cg), prow1 = (uint32_t)(sy1 * (oz + 1) + cg);
   double2 pyc = make_double2(1.0, 0.0);
   if (MODE == kModeInterlaced) pyc = cmul(ph1[y], ph2[colok ? cg : 0]);
   CI u[1][E];
@@ -166,12 +172,14 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
         if (sx1 == sx0) sx1 = -1;
         const T val = (T)p.x;
         if (sx0 >= 0) {
-          if (sy0 >= 0) pk[sx0 * pstride + prow0] = val;
-          if (sy1 >= 0) pk[sx0 * pstride + prow1] = val;
+          const uint32_t r = (uint32_t)sx0 * pstride;
+          if (sy0 >= 0) pk[r + prow0] = val;
+          if (sy1 >= 0) pk[r + prow1] = val;
         }
         if (sx1 >= 0) {
-          if (sy0 >= 0) pk[sx1 * pstride + prow0] = val;
-          if (sy1 >= 0) pk[sx1 * pstride + prow1] = val;
+          const uint32_t r = (uint32_t)sx1 * pstride;
+          if (sy0 >= 0) pk[r + prow0] = val;
+          if (sy1 >= 0) pk[r + prow1] = val;
         }
       }
     }
@@ -183,7 +191,10 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
     for (int s = 0; s < E; ++s) {
       int sx = t + out_m<L>(s) * TCOL + offm;
       if (sx >= L) sx -= L;
-      for (; sx < rs; sx += L) q[((int64_t)sx * n1 + y) * ldq + cg] = u[0][s];
+      for (; sx < rs; sx += L)
+        *reinterpret_cast<CI*>(reinterpret_cast<char*>(q) +
+                               (((uint32_t)sx * (uint32_t)n1 + (uint32_t)y) * (uint32_t)ldq +
+                                (uint32_t)cg) * (uint32_t)sizeof(CI)) = u[0][s];
     }
   }
 }
