@@ -126,6 +126,28 @@ def run_form(lib, corr_mod, ctx, pos, cfg, precision, steps, warmup, profile,
     return out, grids, c
 
 
+def host_grid_rate(lib, corr_mod, grids, cfg, steps):
+    """Correlations/s when the caller hands over host (numpy) grids, as the
+    reference's Python API does: each call uploads both grids over PCIe."""
+    from fftcorr_amd.grid import ConfigSpaceGrid
+    n = cfg["n"]
+    box = cfg["box"]
+    host = []
+    for g in grids:
+        cg = ConfigSpaceGrid((n, n, n), (0, 0, 0), (box, box, box), precision="mixed")
+        g.download(cg.data)
+        host.append(cg)
+    c = corr_mod.PeriodicCorrelator((n, n, n), box / n, cfg["window"], cfg["rmax"], cfg["dr"],
+                                    cfg["kmax"], cfg["dk"], cfg["maxell"], precision="mixed")
+    c.autocorrelate_interlaced(host[0], host[1])
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for h in host:
+            h.data[0, 0, 0] += 0.0  # marks the host copy as written: forces the upload
+        c.autocorrelate_interlaced(host[0], host[1])
+    return steps / (time.perf_counter() - t0)
+
+
 def floor_err(x, ref):
     floor = 1e-3 * np.max(np.abs(ref[0]))
     return np.abs(x - ref) / np.maximum(np.abs(ref), floor)
@@ -181,7 +203,13 @@ def main():
     log("mixed form (headline): %d warmup + %d timed steps" % (args.warmup, args.steps))
     fm, gm, cm = run_form(lib, corr_mod, ctx, pos, cfg, "mixed", args.steps, args.warmup,
                           profile=True)
-    del gm, cm
+    del cm
+    if args.headline_only:
+        del gm
+    else:
+        log("mixed form from host grids (PCIe upload per call): 3 steps")
+        host_rate = host_grid_rate(lib, corr_mod, gm, cfg, 3)
+        del gm
     if args.headline_only:
         log("mixed form: %.4f ms/correlation" % (1e3 * fm["wall_s"] / args.steps))
         return
@@ -229,6 +257,17 @@ def main():
     stage_report = {k: {"ms_per_launch": v[0] / v[1], "launches_per_step": v[1] / args.steps,
                         "GBps": (sb[k] / (v[0] / v[1] * 1e-3) / 1e9) if k in sb else None}
                     for k, v in st.items()}
+
+    # HBM bytes per launch of the dominant kernel, from the committed
+    # FETCH_SIZE / WRITE_SIZE counter runs (profiles/traffic.json), when they
+    # cover that stage.
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        if dom in tj.get("stages", {}):
+            traffic = tj["stages"][dom]["bytes"]
 
     cpu = None
     if not args.no_cpu:
@@ -280,13 +319,14 @@ def main():
         "fp64_plain": side_report(forms["fp64_plain"]),
         "fp64_fused": side_report(forms["fp64_fused"]),
         "fp32_fused": fp32,
+        "mixed_host_grids_pcie_inclusive": {"value": host_rate, "unit": "correlations/s"},
         "mixed_over_fp64_plain": value / rate(forms["fp64_plain"]),
         "mixed_over_fp64_fused": value / rate(forms["fp64_fused"]),
         "fp64_plain_vs_fused_max_floor_rel_err": plain_vs_fused,
         "mass_assign_ms_two_grids": fm["ma_ms"],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": (achieved / PEAK_HBM_GBS) if achieved else None,
-                     "traffic": None},
+                     "traffic": traffic, "algorithmic_bytes": sb.get(dom)},
         "stages": stage_report,
         "cpu_baseline": cpu,
     }

@@ -176,21 +176,25 @@ __global__ void __launch_bounds__(kSegThreads)
   }
 }
 
-// hist[row0 + q][bin] += the chunk partials of the bin in chunk order;
-// counts rows [crow0, crow1) += counts of the bin.
+// One wave per (multipole, bin): hist[row0 + q][bin] += the chunk partials
+// of the bin (lane l sums chunks l, l + 64, ... in order, then a fixed xor
+// tree); counts rows [crow0, crow1) += counts of the bin.
 template <int NV>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kWave)
     k_hist_seg_final(const double* __restrict__ part, const int* __restrict__ chunk_first,
                      const int* __restrict__ bin_counts, int nbins, double* __restrict__ hist,
                      int* __restrict__ counts, int row0, int crow0, int crow1) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x;
+  const int lane = threadIdx.x;
   if (i < NV * nbins) {
     const int q = i / nbins, bin = i - q * nbins;
     double s = 0.0;
-    for (int c = chunk_first[bin]; c < chunk_first[bin + 1]; ++c) s += part[(size_t)c * NV + q];
-    hist[(size_t)(row0 + q) * nbins + bin] += s;
+    for (int c = chunk_first[bin] + lane; c < chunk_first[bin + 1]; c += kWave)
+      s += part[(size_t)c * NV + q];
+    s = wave_sum(s);
+    if (lane == 0) hist[(size_t)(row0 + q) * nbins + bin] += s;
   }
-  if (i < nbins)
+  if (i < nbins && lane == 0)
     for (int r = crow0; r < crow1; ++r) counts[(size_t)r * nbins + i] += bin_counts[i];
 }
 
@@ -251,7 +255,7 @@ void run_hist_seg(G g, const BinnedCells& bc, DeviceBuffer& part, double* hist, 
     FC_LAUNCH_CHECK();
   }
   const int tot = NV * nbins > nbins ? NV * nbins : nbins;
-  hipLaunchKernelGGL((k_hist_seg_final<NV>), dim3((tot + 255) / 256), dim3(256), 0, s,
+  hipLaunchKernelGGL((k_hist_seg_final<NV>), dim3(tot), dim3(kWave), 0, s,
                      part.as<double>(), bc.chunk_first.as<int>(), bc.counts.as<int>(), nbins,
                      hist, counts, row0, crow0, crow1);
   FC_LAUNCH_CHECK();
