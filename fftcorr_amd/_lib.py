@@ -95,6 +95,17 @@ SIGNATURES = {
     "fc_corr_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fc_corr_set_algorithm": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fc_autotune": (ctypes.c_int, [_vp, _ip, ctypes.c_int, ctypes.c_int, _dp]),
+    "fc_batch_create": (ctypes.c_int, [_vp, _ip, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(_vp)]),
+    "fc_batch_destroy": (ctypes.c_int, [_vp]),
+    "fc_batch_grid": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "fc_batch_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "fc_batch_autocorrelate": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "fc_batch_results": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _ip, _dp, _ip, _dp]),
+    "fc_batch_correlator": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "fc_batch_last_time_ms": (ctypes.c_int, [_vp, _dp]),
     "fc_plan_tiles": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _ip]),
     "fc_corr_stage_times": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "fc_hist_accumulate": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,

@@ -259,6 +259,8 @@ struct KEntry {
   FastDiv d2, d1;  // division by ks2 and ks1
   const double *kx, *ky, *kz, *wx, *wy, *wz;
   double coef[NV];
+  int64_t bstride = 0;  // pk elements per batch member
+  __device__ __forceinline__ void offset(int m) { pk += m * bstride; }
   __device__ __forceinline__ void operator()(int64_t pos, double* v) const {
 #pragma clang fp contract(off)
     const uint32_t cell = cells[pos];
@@ -282,6 +284,8 @@ struct WEntry {
   const uint32_t* cells;
   const double* wt;
   int64_t n;
+  int64_t bstride = 0;  // values per batch member
+  __device__ __forceinline__ void offset(int m) { vals += m * bstride; }
   __device__ __forceinline__ void operator()(int64_t pos, double* v) const {
     const double x = (double)vals[cells[pos]];
 #pragma unroll
@@ -294,6 +298,8 @@ template <typename V>
 struct GEntry {
   const V* vals;
   const uint32_t* cells;
+  int64_t bstride = 0;
+  __device__ __forceinline__ void offset(int m) { vals += m * bstride; }
   __device__ __forceinline__ void operator()(int64_t pos, double* v) const {
     v[0] = (double)vals[cells[pos]];
   }
@@ -625,14 +631,25 @@ void ensure_periodic_buffers(fc_corr* c) {
   c->xisub.reserve(sizeof(TI) * (size_t)rs * rs * rs);
 }
 
+// Outputs of one fused x launch: nb batch members, member i reading its
+// spectra at i * a_bstride and writing pk / q1 at i * pk_bstride / q_bstride
+// elements.
+struct PxOut {
+  void* pk;
+  void* q1;
+  int nb = 1;
+  int64_t a_bstride = 0, pk_bstride = 0, q_bstride = 0;
+};
+
 // Register-chained fused x pass (power-of-two x lengths up to 512).
 template <typename T, typename TI, int MODE>
-void launch_periodic_x2(fc_corr* c, const cplx<T>* a, const cplx<T>* b) {
+void launch_periodic_x2(fc_corr* c, const cplx<T>* a, const cplx<T>* b, const PxOut& po) {
   const FftPlan& p = *c->plan;
   const FftPlan& pi = *c->plan_inv;
   constexpr int TC = px_tc<T>();
   const int tilesC = (p.n2h + TC - 1) / TC;
-  const int64_t nblocks = (int64_t)p.n[1] * tilesC;
+  const int64_t per = (int64_t)p.n[1] * tilesC;
+  const int64_t nblocks = per * po.nb;
   if (nblocks > INT32_MAX) fail(FC_ERR_INVALID, "grid too large for the fused x pass");
   const double2* ph[3] = {c->ph[0].as<double2>(), c->ph[1].as<double2>(),
                           c->ph[2].as<double2>()};
@@ -644,8 +661,9 @@ void launch_periodic_x2(fc_corr* c, const cplx<T>* a, const cplx<T>* b) {
                        dim3(px_threads<T, Lv>()), (px_lds_bytes<T, TI, Lv, MODE>()),           \
                        c->ctx->stream, a, b, p.n[1], p.n2h, (int64_t)p.ldc, tilesC,            \
                        p.tw0.as<cplx<T>>(), pi.tw0.as<cplx<TI>>(), ph[0], ph[1], ph[2],        \
-                       c->pk.as<T>(), kslots(c), c->q1.as<cplx<TI>>(), (int64_t)pi.ldc, rs,   \
-                       offm);                                                                 \
+                       static_cast<T*>(po.pk), kslots(c), static_cast<cplx<TI>*>(po.q1),     \
+                       (int64_t)pi.ldc, rs, offm, (int)per, po.a_bstride, po.pk_bstride,      \
+                       po.q_bstride);                                                         \
     break;
   switch (p.n[0]) {
     FC_PX2_CASE(8)
@@ -712,7 +730,7 @@ bool px_fits(const fc_corr* c) {
 template <typename T, typename TI, int MODE>
 void fused_x(fc_corr* c, cplx<T>* w, const cplx<T>* b) {
   if (px_fits(c))
-    launch_periodic_x2<T, TI, MODE>(c, w, b);
+    launch_periodic_x2<T, TI, MODE>(c, w, b, PxOut{c->pk.as<void>(), c->q1.as<void>()});
   else if constexpr (std::is_same<T, TI>::value)
     launch_periodic_x<T, MODE>(c, w, b);
   else
@@ -828,7 +846,7 @@ void periodic_hists(fc_corr* c) {
     KEntry<T, NV> ke{c->kbin.cells.as<uint32_t>(), c->pk.as<T>(), kslots(c),
                      FastDiv((uint32_t)c->kshape[2]), FastDiv((uint32_t)c->kshape[1]),
                      c->kx.as<double>(), c->ky.as<double>(), c->kz.as<double>(),
-                     c->wx.as<double>(), c->wy.as<double>(), c->wz.as<double>(), {}};
+                     c->wx.as<double>(), c->wy.as<double>(), c->wz.as<double>(), {}, 0};
     for (int q = 0; q < NV; ++q) ke.coef[q] = c->kcoef[q];
     run_hist_seg<NV>(ke, c->kbin, c->hs.part_seg, c->k_hist.as<double>(), c->k_counts.as<int>(),
                      0, 0, NV, s);
@@ -1300,6 +1318,302 @@ int fc_corr_last_time_ms(fc_corr* c, double* ms) {
     FC_HIP(hipEventSynchronize(c->ev1));
     float t = 0;
     FC_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    *ms = t;
+  });
+}
+
+}  // extern "C"
+
+// ---- batches of same-geometry periodic correlations -------------------------
+// Every member shares the correlator's geometry, plans, bin-sorted cells and
+// weights; the grids, work arrays and outputs are stacked [member][...] so
+// each pass is one launch over all members.  The launch sequence for a given
+// member count can be captured once as a HIP graph and replayed.
+struct fc_batch {
+  fc_corr* corr = nullptr;
+  int nb = 0;
+  int grid_prec = FC_FP64;
+  size_t grid_bytes = 0;
+  fc::DeviceBuffer input;  // [nb][n0][n1][n2] grid storage
+  std::vector<fc_grid*> grids;
+  fc::DeviceBuffer work, pk, q1, q2, xisub, part;
+  fc::DeviceBuffer r_hist, r_counts, k_hist, k_counts, zerolag;
+  int64_t npk = 0, nq1 = 0, nq2 = 0, nxs = 0;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_n = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+  ~fc_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto g : grids) delete g;
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (corr) fc_corr_destroy(corr);
+  }
+};
+
+namespace fc {
+namespace {
+
+template <typename T, typename TI, int NV>
+void batch_sequence(fc_batch* bt, int n) {
+  fc_corr* c = bt->corr;
+  hipStream_t s = c->ctx->stream;
+  const FftPlan& p = *c->plan;
+  const FftPlan& pi = *c->plan_inv;
+  const int rs = c->rshape, o = rs / 2;
+  const int64_t ldq = pi.ldc;
+  const int64_t hk = (int64_t)c->nell * c->nkbins, hr = (int64_t)c->nell * c->nrbins;
+  fill<int>(bt->r_counts.as<int>(), hr * n, 0, s);
+  fill<double>(bt->r_hist.as<double>(), hr * n, 0.0, s);
+  fill<int>(bt->k_counts.as<int>(), hk * n, 0, s);
+  fill<double>(bt->k_hist.as<double>(), hk * n, 0.0, s);
+  fill<double>(bt->zerolag.as<double>(), n, 0.0, s);
+  cplx<T>* w = bt->work.as<cplx<T>>();
+  const int64_t rows = (int64_t)n * p.n[0] * p.n[1];
+  if (bt->grid_prec == FC_FP64)
+    launch_rows_r2c<T, double>(p, bt->input.as<double>(), p.n[2], w, p.ldc, LoadCopy{}, s, rows);
+  else
+    launch_rows_r2c<T, float>(p, bt->input.as<float>(), p.n[2], w, p.ldc, LoadCopy{}, s, rows);
+  {
+    ColView<T> v = y_view<T>(p, w);
+    v.A = n * p.n[0];
+    launch_cols<T, -1>(p.n[1], v, p.tile[1], p.tw1.as<cplx<T>>(), s);
+  }
+  PxOut po{bt->pk.as<void>(), bt->q1.as<void>(), n, (int64_t)p.n[0] * p.n[1] * p.ldc, bt->npk,
+           bt->nq1};
+  launch_periodic_x2<T, TI, kModeAuto>(c, w, nullptr, po);
+  {
+    ColView<TI> v{bt->q1.as<cplx<TI>>(), (int64_t)pi.n[1] * ldq, ldq, 0, bt->q2.as<cplx<TI>>(),
+                  (int64_t)rs * ldq, ldq, 0, n * rs, 1, pi.n2h, rs, o};
+    launch_cols<TI, +1>(pi.n[1], v, pi.tile[1], pi.tw1.as<cplx<TI>>(), s);
+  }
+  launch_rows_c2r<TI, TI>(pi, bt->q2.as<cplx<TI>>(), ldq, bt->xisub.as<TI>(), rs, s,
+                          (int64_t)n * rs * rs, rs, o);
+  KEntry<T, NV> ke{c->kbin.cells.as<uint32_t>(), bt->pk.as<T>(), kslots(c),
+                   FastDiv((uint32_t)c->kshape[2]), FastDiv((uint32_t)c->kshape[1]),
+                   c->kx.as<double>(), c->ky.as<double>(), c->kz.as<double>(),
+                   c->wx.as<double>(), c->wy.as<double>(), c->wz.as<double>(), {}, bt->npk};
+  for (int q = 0; q < NV; ++q) ke.coef[q] = c->kcoef[q];
+  run_hist_seg<NV>(ke, c->kbin, bt->part, bt->k_hist.as<double>(), bt->k_counts.as<int>(), 0, 0,
+                   NV, s, -1, nullptr, n, hk);
+  WEntry<TI, NV> we{bt->xisub.as<TI>(), c->rbin.cells.as<uint32_t>(), c->rw.as<double>(),
+                    c->rbin.n, bt->nxs};
+  run_hist_seg<NV>(we, c->rbin, bt->part, bt->r_hist.as<double>(), bt->r_counts.as<int>(), 0, 0,
+                   NV, s, c->rbin.tracked_pos, bt->zerolag.as<double>(), n, hr);
+}
+
+template <typename T, typename TI>
+void batch_dispatch(fc_batch* bt, int n) {
+  switch (bt->corr->nell) {
+    case 1:
+      return batch_sequence<T, TI, 1>(bt, n);
+    case 2:
+      return batch_sequence<T, TI, 2>(bt, n);
+    default:
+      return batch_sequence<T, TI, 3>(bt, n);
+  }
+}
+
+void batch_enqueue(fc_batch* bt, int n) {
+  const int prec = bt->corr->prec;
+  if (prec == FC_FP64)
+    batch_dispatch<double, double>(bt, n);
+  else if (prec == FC_FP32)
+    batch_dispatch<float, float>(bt, n);
+  else
+    batch_dispatch<float, double>(bt, n);
+}
+
+template <int NV>
+void batch_weights(fc_corr* c) {
+  ensure_periodic_weights<NV>(c);
+}
+
+}  // namespace
+}  // namespace fc
+
+extern "C" {
+
+int fc_batch_create(fc_ctx* ctx, const int shape[3], double cell_size, int window_correct,
+                    double rmax, double dr, double kmax, double dk, int maxell, int precision,
+                    int nbatch, fc_batch** out) {
+  return fc::guard([&] {
+    fc::check_ctx(ctx);
+    if (!shape || !out) fc::fail(FC_ERR_INVALID, "null argument");
+    if (nbatch < 1) fc::fail(FC_ERR_INVALID, "nbatch must be positive");
+    std::unique_ptr<fc_batch> b(new fc_batch());
+    if (const int rc = fc_corr_create(ctx, FC_PERIODIC, shape, cell_size, nullptr,
+                                      window_correct, rmax, dr, kmax, dk, maxell, precision,
+                                      &b->corr))
+      fc::fail(rc, fc_last_error());
+    fc_corr* c = b->corr;
+    fc::DeviceGuard dg(ctx->device);
+    if (!fc::px_fits(c))
+      fc::fail(FC_ERR_INVALID,
+               "batches need n0 a power of two from 8 to 512 and members under 4 GB");
+    const fc::FftPlan& p = *c->plan;
+    const fc::FftPlan& pi = *c->plan_inv;
+    const size_t csi = pi.csize();
+    const size_t ts = precision == FC_FP64 ? 8 : 4, tsi = precision == FC_FP32 ? 4 : 8;
+    b->nb = nbatch;
+    b->grid_prec = precision == FC_FP64 ? FC_FP64 : FC_FP32;
+    const uint64_t ncell = (uint64_t)shape[0] * shape[1] * shape[2];
+    b->grid_bytes = ncell * (b->grid_prec == FC_FP64 ? 8 : 4);
+    const int rs = c->rshape;
+    b->npk = (int64_t)c->kshape[0] * c->kshape[1] * (c->kshape[2] / 2 + 1);
+    b->nq1 = (int64_t)rs * pi.n[1] * pi.ldc;
+    b->nq2 = (int64_t)rs * rs * pi.ldc;
+    b->nxs = (int64_t)rs * rs * rs;
+    b->input.allocate(b->grid_bytes * nbatch);
+    FC_HIP(hipMemsetAsync(b->input.as<void>(), 0, b->grid_bytes * nbatch, ctx->stream));
+    b->work.allocate(p.complex_bytes() * nbatch);
+    b->pk.allocate(ts * b->npk * nbatch);
+    b->q1.allocate(csi * b->nq1 * nbatch);
+    b->q2.allocate(csi * b->nq2 * nbatch);
+    b->xisub.allocate(tsi * b->nxs * nbatch);
+    const int64_t hk = (int64_t)c->nell * c->nkbins, hr = (int64_t)c->nell * c->nrbins;
+    b->k_hist.allocate(sizeof(double) * hk * nbatch);
+    b->k_counts.allocate(sizeof(int) * hk * nbatch);
+    b->r_hist.allocate(sizeof(double) * hr * nbatch);
+    b->r_counts.allocate(sizeof(int) * hr * nbatch);
+    b->zerolag.allocate(sizeof(double) * nbatch);
+    b->part.allocate(sizeof(double) * 3 * (size_t)std::max(c->kbin.nchunks, c->rbin.nchunks) *
+                     nbatch);
+    switch (c->nell) {
+      case 1:
+        fc::batch_weights<1>(c);
+        break;
+      case 2:
+        fc::batch_weights<2>(c);
+        break;
+      default:
+        fc::batch_weights<3>(c);
+    }
+    fc::ensure_phase_tables(c);
+    for (int i = 0; i < nbatch; ++i) {
+      fc_grid* g = new fc_grid();
+      g->ctx = ctx;
+      for (int d = 0; d < 3; ++d) g->shape[d] = shape[d];
+      g->cell = cell_size;
+      g->prec = b->grid_prec;
+      g->data.wrap(b->input.as<char>() + b->grid_bytes * i, b->grid_bytes);
+      g->batch_view = true;
+      b->grids.push_back(g);
+    }
+    FC_HIP(hipEventCreate(&b->ev0));
+    FC_HIP(hipEventCreate(&b->ev1));
+    FC_HIP(hipStreamSynchronize(ctx->stream));
+    *out = b.release();
+  });
+}
+
+int fc_batch_destroy(fc_batch* b) {
+  return fc::guard([&] {
+    if (!b) return;
+    fc::DeviceGuard dg(b->corr->ctx->device);
+    (void)hipStreamSynchronize(b->corr->ctx->stream);
+    delete b;
+  });
+}
+
+int fc_batch_grid(fc_batch* b, int i, fc_grid** out) {
+  return fc::guard([&] {
+    if (!b || !out) fc::fail(FC_ERR_INVALID, "null argument");
+    if (i < 0 || i >= b->nb) fc::fail(FC_ERR_RANGE, "batch member out of range");
+    *out = b->grids[i];
+  });
+}
+
+int fc_batch_set_graph(fc_batch* b, int on) {
+  return fc::guard([&] {
+    if (!b) fc::fail(FC_ERR_INVALID, "null batch");
+    b->use_graph = on != 0;
+  });
+}
+
+int fc_batch_autocorrelate(fc_batch* b, int n) {
+  return fc::guard([&] {
+    if (!b) fc::fail(FC_ERR_INVALID, "null batch");
+    if (n < 1 || n > b->nb) fc::fail(FC_ERR_RANGE, "member count out of range");
+    fc_corr* c = b->corr;
+    fc::DeviceGuard dg(c->ctx->device);
+    hipStream_t s = c->ctx->stream;
+    FC_HIP(hipEventRecord(b->ev0, s));
+    if (b->use_graph) {
+      if (!b->exec || b->graph_n != n) {
+        if (b->exec) FC_HIP(hipGraphExecDestroy(b->exec));
+        if (b->graph) FC_HIP(hipGraphDestroy(b->graph));
+        b->exec = nullptr;
+        b->graph = nullptr;
+        FC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          fc::batch_enqueue(b, n);
+        } catch (...) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        FC_HIP(hipStreamEndCapture(s, &b->graph));
+        FC_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->graph_n = n;
+      }
+      FC_HIP(hipGraphLaunch(b->exec, s));
+    } else {
+      fc::batch_enqueue(b, n);
+    }
+    FC_HIP(hipEventRecord(b->ev1, s));
+    b->timed = true;
+  });
+}
+
+int fc_batch_results(fc_batch* b, int i, double* r_hist, int* r_counts, double* k_hist,
+                     int* k_counts, double* zerolag) {
+  return fc::guard([&] {
+    if (!b) fc::fail(FC_ERR_INVALID, "null batch");
+    if (i < 0 || i >= b->nb) fc::fail(FC_ERR_RANGE, "batch member out of range");
+    fc_corr* c = b->corr;
+    fc::DeviceGuard dg(c->ctx->device);
+    hipStream_t s = c->ctx->stream;
+    const size_t hk = (size_t)c->nell * c->nkbins, hr = (size_t)c->nell * c->nrbins;
+    if (r_hist)
+      FC_HIP(hipMemcpyAsync(r_hist, b->r_hist.as<double>() + hr * i, sizeof(double) * hr,
+                            hipMemcpyDeviceToHost, s));
+    if (r_counts)
+      FC_HIP(hipMemcpyAsync(r_counts, b->r_counts.as<int>() + hr * i, sizeof(int) * hr,
+                            hipMemcpyDeviceToHost, s));
+    if (k_hist)
+      FC_HIP(hipMemcpyAsync(k_hist, b->k_hist.as<double>() + hk * i, sizeof(double) * hk,
+                            hipMemcpyDeviceToHost, s));
+    if (k_counts)
+      FC_HIP(hipMemcpyAsync(k_counts, b->k_counts.as<int>() + hk * i, sizeof(int) * hk,
+                            hipMemcpyDeviceToHost, s));
+    if (zerolag)
+      FC_HIP(hipMemcpyAsync(zerolag, b->zerolag.as<double>() + i, sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    FC_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int fc_batch_correlator(fc_batch* b, fc_corr** out) {
+  return fc::guard([&] {
+    if (!b || !out) fc::fail(FC_ERR_INVALID, "null argument");
+    *out = b->corr;
+  });
+}
+
+int fc_batch_last_time_ms(fc_batch* b, double* ms) {
+  return fc::guard([&] {
+    if (!b || !ms) fc::fail(FC_ERR_INVALID, "null argument");
+    if (!b->timed) fc::fail(FC_ERR_RUNTIME, "no batch has run yet");
+    fc::DeviceGuard dg(b->corr->ctx->device);
+    FC_HIP(hipEventSynchronize(b->ev1));
+    float t = 0;
+    FC_HIP(hipEventElapsedTime(&t, b->ev0, b->ev1));
     *ms = t;
   });
 }

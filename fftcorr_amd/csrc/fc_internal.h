@@ -66,17 +66,20 @@ class DeviceBuffer {
   ~DeviceBuffer() { release(); }
   DeviceBuffer(const DeviceBuffer&) = delete;
   DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) {
+  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_), owned_(o.owned_) {
     o.ptr_ = nullptr;
     o.bytes_ = 0;
+    o.owned_ = true;
   }
   DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
     if (this != &o) {
       release();
       ptr_ = o.ptr_;
       bytes_ = o.bytes_;
+      owned_ = o.owned_;
       o.ptr_ = nullptr;
       o.bytes_ = 0;
+      o.owned_ = true;
     }
     return *this;
   }
@@ -91,9 +94,17 @@ class DeviceBuffer {
     if (bytes > bytes_) allocate(bytes);
   }
   void release() {
-    if (ptr_) (void)hipFree(ptr_);
+    if (ptr_ && owned_) (void)hipFree(ptr_);
     ptr_ = nullptr;
     bytes_ = 0;
+    owned_ = true;
+  }
+  // Non-owning view of memory owned elsewhere (a slice of a batch buffer).
+  void wrap(void* p, size_t bytes) {
+    release();
+    ptr_ = p;
+    bytes_ = bytes;
+    owned_ = false;
   }
   template <typename T = void>
   T* as() const {
@@ -104,6 +115,7 @@ class DeviceBuffer {
  private:
   void* ptr_ = nullptr;
   size_t bytes_ = 0;
+  bool owned_ = true;
 };
 
 }  // namespace fc

@@ -12,6 +12,7 @@ struct fc_grid {
   double cell = 1.0;
   int prec = FC_FP64;  // storage: FC_FP64 -> double, FC_FP32 -> float
   fc::DeviceBuffer data;
+  bool batch_view = false;  // data is a slice of a batch's input buffer
   uint64_t size() const { return (uint64_t)shape[0] * shape[1] * shape[2]; }
   size_t elem() const { return prec == FC_FP64 ? sizeof(double) : sizeof(float); }
 };

@@ -73,10 +73,13 @@ constexpr size_t px_lds_bytes() {
   return fwd > inv ? fwd : inv;
 }
 
-// One workgroup per (y row, tile of TC kz columns) of the work grid
-// [n0][n1][ldc]; tiles are dealt so that each of the 8 XCDs (which receive
-// consecutive workgroups in turn) runs a contiguous range of tiles and the two
-// tiles sharing a 128-byte line meet in one L2.
+// One workgroup per (batch member, y row, tile of TC kz columns) of the work
+// grids [batch][n0][n1][ldc]; tiles are dealt so that each of the 8 XCDs
+// (which receive consecutive workgroups in turn) runs a contiguous range of
+// tiles and the two tiles sharing a 128-byte line meet in one L2.  Batch
+// member b reads A, B at b * a_bstride and writes pk, q at b * pk_bstride,
+// b * q_bstride (elements).  In cross mode B is the one shared grid-2
+// spectrum (a_bstride does not apply to it).
 //   A: grid a after its z and y passes (forward precision T).
 //   B: grid b likewise (interlaced) or the fully transformed grid 2 (cross).
 //   pk: compact k-subgrid values (T); q[sx][y][c] (row stride ldq, inverse
@@ -91,7 +94,8 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
                   const cplx<TI>* __restrict__ twi, const double2* __restrict__ ph0,
                   const double2* __restrict__ ph1, const double2* __restrict__ ph2,
                   T* __restrict__ pk, KSlots ks, cplx<TI>* __restrict__ q, int64_t ldq, int rs,
-                  int offm) {
+                  int offm, int tiles_per_batch, int64_t a_bstride, int64_t pk_bstride,
+                  int64_t q_bstride) {
   using C = cplx<T>;
   using CI = cplx<TI>;
   constexpr int E = FftCfg<L>::E;
@@ -104,7 +108,13 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
   C* lds = reinterpret_cast<C*>(smem_raw);
   CI* ldsi = reinterpret_cast<CI*>(smem_raw);
 
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile_all = xcd_tile(blockIdx.x, gridDim.x);
+  const int bat = tile_all / tiles_per_batch;
+  const int tile = tile_all - bat * tiles_per_batch;
+  A += bat * a_bstride;
+  if (MODE == kModeInterlaced) B += bat * a_bstride;
+  pk += bat * pk_bstride;
+  q += bat * q_bstride;
   const int y = tile / tilesC;
   const int c0 = (tile - y * tilesC) * TC;
   const int ncol = min(TC, n2h - c0);

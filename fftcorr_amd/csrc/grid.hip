@@ -135,6 +135,8 @@ int fc_grid_create(fc_ctx* ctx, const int shape[3], const double posmin[3], doub
 int fc_grid_destroy(fc_grid* grid) {
   return fc::guard([&] {
     if (!grid) return;
+    if (grid->batch_view)
+      fc::fail(FC_ERR_INVALID, "this grid belongs to a batch; destroy the batch instead");
     fc::DeviceGuard dg(grid->ctx->device);
     (void)hipStreamSynchronize(grid->ctx->stream);
     delete grid;

@@ -144,12 +144,17 @@ struct FastDiv {
 // Segmented histogram over bin-sorted entries: workgroup b sums the NV values
 // g(pos, v) of the entries of chunk b (each thread in entry order, then a
 // fixed tree over lanes and waves) into part[b][NV].  When pos == zpos the
-// first value is also stored to *zval (the zero-lag cell).
+// first value is also stored to *zval (the zero-lag cell).  blockIdx.y is
+// the batch member: g.offset(m) selects its values, and its partials and
+// zero-lag value go to part[m][nchunks][NV] and zval[m].
 template <int NV, class G>
 __global__ void __launch_bounds__(kSegThreads)
     k_hist_seg(G g, const int4* __restrict__ chunks, double* __restrict__ part, int64_t zpos,
                double* __restrict__ zval) {
   __shared__ double sw[kSegThreads / kWave][NV];
+  g.offset(blockIdx.y);
+  part += (size_t)blockIdx.y * gridDim.x * NV;
+  if (zval) zval += blockIdx.y;
   const int4 ch = chunks[blockIdx.x];
   double acc[NV];
 #pragma unroll
@@ -183,8 +188,12 @@ template <int NV>
 __global__ void __launch_bounds__(kWave)
     k_hist_seg_final(const double* __restrict__ part, const int* __restrict__ chunk_first,
                      const int* __restrict__ bin_counts, int nbins, double* __restrict__ hist,
-                     int* __restrict__ counts, int row0, int crow0, int crow1) {
+                     int* __restrict__ counts, int row0, int crow0, int crow1, int nchunks,
+                     int64_t out_bstride) {
   const int i = blockIdx.x;
+  part += (size_t)blockIdx.y * nchunks * NV;
+  hist += blockIdx.y * out_bstride;
+  counts += blockIdx.y * out_bstride;
   const int lane = threadIdx.x;
   if (i < NV * nbins) {
     const int q = i / nbins, bin = i - q * nbins;
@@ -243,21 +252,23 @@ struct HistScratch {
   }
 };
 
+// nb batch members: member m's histogram rows start at hist + m * out_bstride
+// (counts likewise).
 template <int NV, class G>
 void run_hist_seg(G g, const BinnedCells& bc, DeviceBuffer& part, double* hist, int* counts,
                   int row0, int crow0, int crow1, hipStream_t s, int64_t zpos = -1,
-                  double* zval = nullptr) {
+                  double* zval = nullptr, int nb = 1, int64_t out_bstride = 0) {
   const int nbins = bc.nbins;
   if (bc.nchunks > 0) {
-    part.reserve(sizeof(double) * (size_t)bc.nchunks * NV);
-    hipLaunchKernelGGL((k_hist_seg<NV, G>), dim3(bc.nchunks), dim3(kSegThreads), 0, s, g,
+    part.reserve(sizeof(double) * (size_t)bc.nchunks * NV * nb);
+    hipLaunchKernelGGL((k_hist_seg<NV, G>), dim3(bc.nchunks, nb), dim3(kSegThreads), 0, s, g,
                        bc.chunks.as<int4>(), part.as<double>(), zpos, zval);
     FC_LAUNCH_CHECK();
   }
   const int tot = NV * nbins > nbins ? NV * nbins : nbins;
-  hipLaunchKernelGGL((k_hist_seg_final<NV>), dim3(tot), dim3(kWave), 0, s,
+  hipLaunchKernelGGL((k_hist_seg_final<NV>), dim3(tot, nb), dim3(kWave), 0, s,
                      part.as<double>(), bc.chunk_first.as<int>(), bc.counts.as<int>(), nbins,
-                     hist, counts, row0, crow0, crow1);
+                     hist, counts, row0, crow0, crow1, bc.nchunks, out_bstride);
   FC_LAUNCH_CHECK();
 }
 

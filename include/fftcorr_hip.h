@@ -53,6 +53,7 @@ extern "C" {
 typedef struct fc_ctx fc_ctx;
 typedef struct fc_grid fc_grid;
 typedef struct fc_corr fc_corr;
+typedef struct fc_batch fc_batch;
 
 /* ---- runtime ---------------------------------------------------------- */
 const char* fc_last_error(void);
@@ -165,6 +166,33 @@ int fc_corr_set_profiling(fc_corr* corr, int on);
  * FC_FP64 or FC_FP32. */
 int fc_corr_set_algorithm(fc_corr* corr, int algo);
 int fc_corr_stage_times(fc_corr* corr, double* ms, int64_t* launches);
+
+/* ---- Batches of periodic autocorrelations (no reference counterpart: the
+ *      reference correlates one grid per call, correlator.pyx:72-184; this
+ *      runs many same-geometry mocks per launch) ---------------------------- */
+/* nbatch member grids [shape] stored back to back on the device, one shared
+ * PeriodicCorrelator geometry (arguments as fc_corr_create).  Needs shape[0]
+ * a power of two from 8 to 512. */
+int fc_batch_create(fc_ctx* ctx, const int shape[3], double cell_size, int window_correct,
+                    double rmax, double dr, double kmax, double dk, int maxell, int precision,
+                    int nbatch, fc_batch** out);
+int fc_batch_destroy(fc_batch* batch);
+/* Member i's density grid (owned by the batch; fill it with fc_mass_assign,
+ * fc_grid_upload, fc_grid_normalize ...; fc_grid_destroy on it fails). */
+int fc_batch_grid(fc_batch* batch, int i, fc_grid** out);
+/* 1: capture the launch sequence of fc_batch_autocorrelate as a HIP graph the
+ * first time a member count is used and replay it afterwards. */
+int fc_batch_set_graph(fc_batch* batch, int on);
+/* Autocorrelates members 0 .. n-1 (PeriodicCorrelator::autocorrelate each). */
+int fc_batch_autocorrelate(fc_batch* batch, int n);
+/* Member i's histograms ([n_ell][nbins] each; bin centres and geometry from
+ * fc_batch_correlator + fc_corr_results / fc_corr_geometry). */
+int fc_batch_results(fc_batch* batch, int i, double* r_hist, int* r_counts, double* k_hist,
+                     int* k_counts, double* zerolag);
+/* The batch's shared correlator (geometry, bin centres; owned by the batch). */
+int fc_batch_correlator(fc_batch* batch, fc_corr** out);
+/* Device time of the last fc_batch_autocorrelate. */
+int fc_batch_last_time_ms(fc_batch* batch, double* ms);
 
 /* ---- HistogramList: cc/histogram/histogram_list.h:10-67,
  *      fftcorr/histogram/histogram_list.pyx:9-44 -------------------------- */
