@@ -148,9 +148,162 @@ def host_grid_rate(lib, corr_mod, grids, cfg, steps):
     return steps / (time.perf_counter() - t0)
 
 
+BATCH_CFG = dict(n=256, box=1000.0, parents=10_000, children=100, sigma=4.0, mocks=64,
+                 rmax=200.0, dr=5.0, kmax=0.03, dk=0.01, maxell=4, window=2, scheme=1)
+
+
+def batched_mocks(lib, corr_mod, ctx, reps):
+    """64 seeded Thomas mocks (10,000 parents x Poisson(100), sigma 4 Mpc/h,
+    1000 Mpc/h periodic box), TSC onto 256^3, TSC-aliased window, xi_l and
+    P_l (kmax 0.03), l <= 4: mixed-precision batch (one HIP graph per batch)
+    against the FP64 plain form run mock by mock on the same grids."""
+    from fftcorr_amd.catalog import thomas_catalog
+    b = BATCH_CFG
+    n = b["n"]
+    cell = b["box"] / n
+    args = ((n, n, n), cell, b["window"], b["rmax"], b["dr"], b["kmax"], b["dk"], b["maxell"])
+    batch = corr_mod.PeriodicBatch(*args, nbatch=b["mocks"], precision="mixed", use_graph=True)
+    t_ma = 0.0
+    for i in range(b["mocks"]):
+        pos = thomas_catalog(b["parents"], b["children"], b["sigma"], 0.0, b["box"], 1000 + i)
+        t0 = time.perf_counter()
+        batch.assign(i, pos, scheme=b["scheme"])
+        t_ma += time.perf_counter() - t0
+    ctx.synchronize()
+    batch.autocorrelate()  # captures the graph
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.autocorrelate()
+    ctx.synchronize()
+    rate = reps * b["mocks"] / (time.perf_counter() - t0)
+    # FP64 plain, mock by mock, on the same grids (downloaded once).
+    plain = corr_mod.PeriodicCorrelator(*args, precision="fp64")
+    plain.algorithm = "plain"
+    g = lib.DeviceGrid((n, n, n), (0, 0, 0), cell, "fp64", ctx)
+    nplain = 8
+    hosts = [batch.grid(i).download() for i in range(nplain)]
+    err = 0.0
+    t_plain = 0.0
+    for i in range(nplain):
+        g.upload(hosts[i])
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        lib.check(lib.lib().fc_corr_autocorrelate(plain._handle, g.handle))
+        ctx.synchronize()
+        t_plain += time.perf_counter() - t0
+        plain._fetch()
+        ref = plain.results()
+        got = batch.results(i)
+        err = max(err, float(multipole_err(got, ref, "k_hist").max()),
+                  float(multipole_err(got, ref, "r_hist").max()))
+    plain_rate = nplain / t_plain
+    return {"value": rate, "unit": "mocks/s", "mocks": b["mocks"], "grid": n,
+            "particles_per_mock": b["parents"] * b["children"],
+            "fp64_plain_mock_by_mock": plain_rate, "batched_over_fp64_plain": rate / plain_rate,
+            "max_floor_rel_err_vs_fp64_plain": err,
+            "mass_assign_ms_per_mock": 1e3 * t_ma / b["mocks"]}
+
+
+SERVE_CFG = dict(rate=20.0, jobs=40, seed=5, box=1000.0, sizes=(128, 256, 384, 512),
+                 npart=(1e5, 1e7), slots=4)
+
+
+def serving(lib, corr_mod):
+    """Poisson arrivals (20 jobs/s, seed 5) of periodic xi_l / P_l jobs, l <= 4:
+    grids U{128, 256, 384, 512}^3, log-uniform 0.1M-10M uniform or Thomas
+    points (parents N/100, sigma 4) in a 1000 Mpc/h box, float32 weights
+    U[0.5, 1.5], CIC; served by 4 slots in the mixed form with resident
+    plans.  Reports latency percentiles under the stream, the burst capacity
+    (all jobs queued at once), and the FP64 plain form's capacity on the same
+    jobs run one after another."""
+    from fftcorr_amd.catalog import thomas_catalog, uniform_catalog
+    from fftcorr_amd.serve import CorrelationServer, Job, poisson_stream
+    c = SERVE_CFG
+    rng = np.random.default_rng(c["seed"])
+    jobs = []
+    for i in range(c["jobs"]):
+        n = int(rng.choice(c["sizes"]))
+        npart = int(round(np.exp(rng.uniform(np.log(c["npart"][0]), np.log(c["npart"][1])))))
+        if rng.uniform() < 0.5:
+            pos = uniform_catalog(npart, 0.0, c["box"], c["seed"] * 1000 + i)
+        else:
+            pos = thomas_catalog(max(1, npart // 100), 100, 4.0, 0.0, c["box"],
+                                 c["seed"] * 1000 + i)
+        w = rng.uniform(0.5, 1.5, size=pos.shape[0]).astype(np.float32)
+        if rng.uniform() < 0.5:
+            kw = dict(rmax=200.0, dr=5.0, kmax=0.03, dk=0.01)
+        else:
+            kw = dict(rmax=20.0, dr=5.0, kmax=0.4, dk=0.01)
+        jobs.append(Job(n=n, box=c["box"], positions=pos, weights=w, tag=i, **kw))
+    arrivals = poisson_stream(c["rate"], c["jobs"], c["seed"])
+    with CorrelationServer(slots=c["slots"], precision="mixed") as srv:
+        # Warm every slot with every geometry (resident plans and subgrids).
+        geos = {}
+        for j in jobs:
+            geos.setdefault((j.n, j.kmax), j)
+        warm = [Job(n=j.n, box=j.box, positions=j.positions[:1000], rmax=j.rmax, dr=j.dr,
+                    kmax=j.kmax, dk=j.dk) for j in geos.values() for _ in range(c["slots"])]
+        for f in [srv.submit(j) for j in warm]:
+            f.result()
+        t0 = time.perf_counter()
+        futs = []
+        for j, t in zip(jobs, arrivals):
+            dt = t0 + t - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            futs.append(srv.submit(j))
+        res = [f.result() for f in futs]
+        lat = np.array([r.latency_ms for r in res])
+        tb = time.perf_counter()
+        for f in [srv.submit(j) for j in jobs]:
+            f.result()
+        burst = len(jobs) / (time.perf_counter() - tb)
+    # FP64 plain, one job after another (correlators cached per geometry).
+    cache = {}
+    tp = 0.0
+    for j in jobs:
+        key = (j.n, j.kmax)
+        if key not in cache:
+            cc = corr_mod.PeriodicCorrelator((j.n,) * 3, j.box / j.n, j.window, j.rmax, j.dr,
+                                             j.kmax, j.dk, j.maxell, precision="fp64")
+            cc.algorithm = "plain"
+            g = lib.DeviceGrid((j.n,) * 3, (0, 0, 0), j.box / j.n, "fp64")
+            cache[key] = (cc, g)
+        cc, g = cache[key]
+        pos = np.ascontiguousarray(j.positions)
+        w = np.ascontiguousarray(j.weights, np.float64)
+        t1 = time.perf_counter()
+        g.clear()
+        st = lib.MaStats()
+        lib.check(lib.lib().fc_mass_assign(g.handle, lib.dptr(pos), pos.shape[0], 3, lib.dptr(w),
+                                           1.0, j.scheme, 1, 0.0, ctypes.byref(st)))
+        g.normalize(st.totw / float(j.n) ** 3)
+        lib.check(lib.lib().fc_corr_autocorrelate(cc._handle, g.handle))
+        cc._fetch()
+        tp += time.perf_counter() - t1
+    return {"jobs": len(jobs), "arrival_rate_hz": c["rate"], "slots": c["slots"],
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "throughput_jobs_per_s": len(jobs) / (max(r.finished for r in res) - t0),
+            "burst_capacity_jobs_per_s": burst,
+            "fp64_plain_serial_jobs_per_s": len(jobs) / tp,
+            "burst_over_fp64_plain": burst / (len(jobs) / tp)}
+
+
 def floor_err(x, ref):
+    """|x - ref| / max(|ref|, 1e-3 max|ref_l=0|) per multipole row and bin."""
     floor = 1e-3 * np.max(np.abs(ref[0]))
     return np.abs(x - ref) / np.maximum(np.abs(ref), floor)
+
+
+def multipole_err(res, ref, key):
+    """The error metric on the multipoles themselves (histogram / count, as
+    the output tables hold them), bins with counts only; counts are equal."""
+    c = key.replace("hist", "counts")
+    ok = ref[c][0] > 0
+    x = res[key][:, ok] / res[c][:, ok]
+    r = ref[key][:, ok] / ref[c][:, ok]
+    return floor_err(x, r)
 
 
 def cpu_baseline(grids, cfg):
@@ -232,10 +385,10 @@ def main():
     value = rate(fm)
     ref64 = forms["fp64_fused"]["res"]
 
-    def errs(res):
+    def errs(res, sums=False):
         err = {}
         for key, tag in (("k_hist", "P"), ("r_hist", "xi")):
-            e = floor_err(res[key], ref64[key])
+            e = floor_err(res[key], ref64[key]) if sums else multipole_err(res, ref64, key)
             for q in range(e.shape[0]):
                 err["%s_%d" % (tag, 2 * q)] = float(e[q].max())
         return err
@@ -280,6 +433,11 @@ def main():
                          "numpy/scipy oracle (scipy.fft workers=%d), %.1f s" % (threads, dt),
                "gpu_fp64_vs_cpu_max_floor_rel_err": e_cpu}
 
+    log("batched mocks: 64 x 256^3 TSC, mixed, HIP graph")
+    batch_report = batched_mocks(lib, corr_mod, ctx, 5)
+    log("serving: Poisson stream of mixed-size jobs")
+    serve_report = serving(lib, corr_mod)
+
     lat = fm["lat_ms"]
 
     def side_report(out):
@@ -315,6 +473,7 @@ def main():
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
         "max_floor_rel_err_vs_fp64": err,
+        "max_floor_rel_err_vs_fp64_on_hist_sums": errs(fm["res"], sums=True),
         "counts_bit_exact": counts_equal(fm["res"]),
         "fp64_plain": side_report(forms["fp64_plain"]),
         "fp64_fused": side_report(forms["fp64_fused"]),
@@ -324,6 +483,8 @@ def main():
         "mixed_over_fp64_fused": value / rate(forms["fp64_fused"]),
         "fp64_plain_vs_fused_max_floor_rel_err": plain_vs_fused,
         "mass_assign_ms_two_grids": fm["ma_ms"],
+        "batched_256": batch_report,
+        "serving": serve_report,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": (achieved / PEAK_HBM_GBS) if achieved else None,
                      "traffic": traffic, "algorithmic_bytes": sb.get(dom)},

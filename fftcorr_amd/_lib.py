@@ -59,6 +59,8 @@ SIGNATURES = {
     "fc_last_error": (ctypes.c_char_p, []),
     "fc_version": (ctypes.c_int, []),
     "fc_device_count": (ctypes.c_int, [_ip]),
+    "fc_device_memory": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_size_t),
+                                        ctypes.POINTER(ctypes.c_size_t)]),
     "fc_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     "fc_ctx_destroy": (ctypes.c_int, [_vp]),
     "fc_ctx_synchronize": (ctypes.c_int, [_vp]),

@@ -9,7 +9,8 @@ Same constructors, ``from_grid_spec``, ``autocorrelate``, ``cross_correlate``,
     ("fp32" with the pruned inverse FFT in FP64);
   * ``algorithm``: "fused" (default; fused x pass, pruned inverse) or
     "plain" (the reference's stage sequence on the device, fp64/fp32 only);
-  * ``device``: the GPU to run on;
+  * ``device``: the GPU to run on; ``context``: an explicit fftcorr_amd
+    context (its own HIP stream), used by the serving layer;
   * ``PeriodicCorrelator.autocorrelate_interlaced(grid_a, grid_b)`` for a
     pair of grids assigned with shift 0 and 0.5 cells;
   * ``zerolag`` and ``last_time_ms`` properties.
@@ -30,11 +31,11 @@ class BaseCorrelator:
     _kind = None
 
     def _init(self, shape, cell_size, posmin, window_correct, rmax, dr, kmax, dk, maxell,
-              fftw_flags, precision, device):
+              fftw_flags, precision, device, context=None):
         self._shape = tuple(int(s) for s in np.ascontiguousarray(shape, dtype=np.intc))
         if len(self._shape) != 3:
             raise ValueError("shape must have 3 entries")
-        self._ctx = _lib.context(device)
+        self._ctx = context if context is not None else _lib.context(device)
         self._precision = _lib.precision_code(precision)
         self._fftw_flags = int(fftw_flags)
         s = np.ascontiguousarray(self._shape, np.intc)
@@ -183,9 +184,9 @@ class PeriodicCorrelator(BaseCorrelator):
     _kind = _lib.FC_PERIODIC
 
     def __init__(self, shape, cell_size, window_correct, rmax, dr, kmax, dk, maxell,
-                 fftw_flags=0, precision="fp64", device=None):
+                 fftw_flags=0, precision="fp64", device=None, context=None):
         self._init(shape, cell_size, None, window_correct, rmax, dr, kmax, dk, maxell,
-                   fftw_flags, precision, device)
+                   fftw_flags, precision, device, context)
 
     @classmethod
     def from_grid_spec(cls, grid, rmax, dr, kmax, dk, maxell, window_correct=0, fftw_flags=0,

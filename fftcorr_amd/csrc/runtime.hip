@@ -23,6 +23,17 @@ int fc_device_count(int* count) {
   });
 }
 
+int fc_device_memory(int device, size_t* free_bytes, size_t* total_bytes) {
+  return fc::guard([&] {
+    if (!free_bytes || !total_bytes) fc::fail(FC_ERR_INVALID, "null argument");
+    int n = 0;
+    FC_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) fc::fail(FC_ERR_RANGE, "no such device");
+    fc::DeviceGuard dg(device);
+    FC_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  });
+}
+
 int fc_ctx_create(int device, fc_ctx** out) {
   return fc::guard([&] {
     if (!out) fc::fail(FC_ERR_INVALID, "null output handle");

@@ -21,6 +21,7 @@
 #ifndef FFTCORR_HIP_H
 #define FFTCORR_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -59,6 +60,9 @@ typedef struct fc_batch fc_batch;
 const char* fc_last_error(void);
 int fc_version(void);
 int fc_device_count(int* count);
+/* Free and total device memory in bytes (admission control of the serving
+ * layer; no reference counterpart). */
+int fc_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
 /* One device + one HIP stream; all handles made from it run on that stream. */
 int fc_ctx_create(int device, fc_ctx** out);
 int fc_ctx_destroy(fc_ctx* ctx);
