@@ -126,7 +126,7 @@ struct fc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   // Scratch reused across calls on this context (sort temp storage, staging).
-  fc::DeviceBuffer scratch[4];
+  fc::DeviceBuffer scratch[12];  // per-context work buffers, grown on demand
   int arch_cus = 0;
 };
 

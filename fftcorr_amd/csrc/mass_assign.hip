@@ -8,7 +8,9 @@
 //                     cell key, in-cell offsets, FP64 weight sums per block;
 //   2. radix sort of (cell key, particle index) — stable, so the order is
 //      fixed by the input order;
-//   3. k_row_starts   first sorted particle of every (x, y) row of cells;
+//   3. k_cell_starts  first sorted particle of every cell (a dense table,
+//                     each entry written once, by the particle that opens the
+//                     cell or the next non-empty one);
 //   4. k_ma_gather    one thread per target cell sums, in FP64 and in a fixed
 //                     order (27 stencil offsets, then sorted particles), the
 //                     contributions of the particles in its neighbour cells.
@@ -125,24 +127,18 @@ __global__ void k_permute(const uint32_t* __restrict__ order, const double4* __r
     out[i] = rec[order[i]];
 }
 
-__device__ __forceinline__ int64_t lower_bound(const uint64_t* __restrict__ k, int64_t lo,
-                                               int64_t hi, uint64_t key) {
-  while (lo < hi) {
-    const int64_t mid = lo + ((hi - lo) >> 1);
-    if (k[mid] < key)
-      lo = mid + 1;
-    else
-      hi = mid;
+// start[c] = first sorted particle with key >= c, c in [0, ncells].  Sorted
+// particle i (i <= nvalid; particle nvalid stands for the end, key ncells)
+// writes the entries (key[i-1], key[i]].
+__global__ void k_cell_starts(const uint64_t* __restrict__ keys, const unsigned long long* nvalid_p,
+                              uint64_t ncells, int32_t* __restrict__ start) {
+  const int64_t nvalid = (int64_t)*nvalid_p;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= nvalid;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t prev = i == 0 ? -1 : (int64_t)keys[i - 1];
+    const int64_t cur = i == nvalid ? (int64_t)ncells : (int64_t)keys[i];
+    for (int64_t c = prev + 1; c <= cur; ++c) start[c] = (int32_t)i;
   }
-  return lo;
-}
-
-// row_start[r] = first sorted particle with key >= r*n2, r in [0, nrows].
-__global__ void k_row_starts(const uint64_t* __restrict__ keys, int64_t n, int64_t nrows,
-                             int n2, int64_t* __restrict__ row_start) {
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= nrows;
-       r += (int64_t)gridDim.x * blockDim.x)
-    row_start[r] = lower_bound(keys, 0, n, (uint64_t)r * n2);
 }
 
 // 1D stencil weight for offset o (target = source + o) and in-cell offset r.
@@ -162,12 +158,11 @@ __device__ __forceinline__ double stencil(int o, double r) {
 
 template <typename Tg, int SCHEME>
 __global__ void __launch_bounds__(TX* TY* TZ)
-    k_ma_gather(Tg* __restrict__ grid, GridGeom g, const uint64_t* __restrict__ keys,
-                const double4* __restrict__ srec, const int64_t* __restrict__ row_start,
-                int tiles_y, int tiles_z) {
+    k_ma_gather(Tg* __restrict__ grid, GridGeom g, const int32_t* __restrict__ start,
+                const double4* __restrict__ srec, int tiles_y, int tiles_z) {
 #pragma clang fp contract(off)
-  __shared__ int64_t cs[HX * HY * HZ];
-  __shared__ int64_t ce[HX * HY * HZ];
+  __shared__ int32_t cs[HX * HY * HZ];
+  __shared__ int32_t ce[HX * HY * HZ];
   int b = blockIdx.x;
   const int bz = b % tiles_z;
   b /= tiles_z;
@@ -179,12 +174,9 @@ __global__ void __launch_bounds__(TX* TY* TZ)
     const int gx = ((x0 + hx - 1) % g.n0 + g.n0) % g.n0;
     const int gy = ((y0 + hy - 1) % g.n1 + g.n1) % g.n1;
     const int gz = ((z0 + hz - 1) % g.n2 + g.n2) % g.n2;
-    const int64_t r = (int64_t)gx * g.n1 + gy;
-    const int64_t rs = row_start[r], re = row_start[r + 1];
-    const uint64_t key = (uint64_t)r * g.n2 + gz;
-    const int64_t s = lower_bound(keys, rs, re, key);
-    cs[h] = s;
-    ce[h] = lower_bound(keys, s, re, key + 1);
+    const int64_t c = ((int64_t)gx * g.n1 + gy) * g.n2 + gz;
+    cs[h] = start[c];
+    ce[h] = start[c + 1];
   }
   __syncthreads();
   const int tz = threadIdx.x % TZ, ty = (threadIdx.x / TZ) % TY, tx = threadIdx.x / (TZ * TY);
@@ -195,14 +187,14 @@ __global__ void __launch_bounds__(TX* TY* TZ)
   if (SCHEME == 0) {  // NGP: only the cell itself
     const int h = ((tx + 1) * HY + (ty + 1)) * HZ + (tz + 1);
     touched = cs[h] < ce[h];
-    for (int64_t p = cs[h]; p < ce[h]; ++p) sum += srec[p].w;
+    for (int p = cs[h]; p < ce[h]; ++p) sum += srec[p].w;
   } else {
     for (int ox = -1; ox <= 1; ++ox)
       for (int oy = -1; oy <= 1; ++oy)
         for (int oz = -1; oz <= 1; ++oz) {
           const int h = ((tx - ox + 1) * HY + (ty - oy + 1)) * HZ + (tz - oz + 1);
           touched |= cs[h] < ce[h];
-          for (int64_t p = cs[h]; p < ce[h]; ++p) {
+          for (int p = cs[h]; p < ce[h]; ++p) {
             const double4 q = srec[p];
             const double wx = stencil<SCHEME>(ox, q.x) * q.w;
             const double wy = stencil<SCHEME>(oy, q.y);
@@ -218,24 +210,21 @@ __global__ void __launch_bounds__(TX* TY* TZ)
 }
 
 template <typename Tg>
-void launch_gather(int scheme, Tg* grid, const GridGeom& gg, const uint64_t* keys,
-                   const double4* srec, const int64_t* row_start, hipStream_t s) {
+void launch_gather(int scheme, Tg* grid, const GridGeom& gg, const int32_t* start,
+                   const double4* srec, hipStream_t s) {
   const int tx = (gg.n0 + TX - 1) / TX, ty = (gg.n1 + TY - 1) / TY, tz = (gg.n2 + TZ - 1) / TZ;
   const uint64_t nb = (uint64_t)tx * ty * tz;
   if (nb > 0x7fffffffULL) fail(FC_ERR_INVALID, "grid too large for mass assignment");
   dim3 grd((unsigned)nb), blk(TX * TY * TZ);
   switch (scheme) {
     case 0:
-      hipLaunchKernelGGL((k_ma_gather<Tg, 0>), grd, blk, 0, s, grid, gg, keys, srec, row_start,
-                         ty, tz);
+      hipLaunchKernelGGL((k_ma_gather<Tg, 0>), grd, blk, 0, s, grid, gg, start, srec, ty, tz);
       break;
     case 1:
-      hipLaunchKernelGGL((k_ma_gather<Tg, 1>), grd, blk, 0, s, grid, gg, keys, srec, row_start,
-                         ty, tz);
+      hipLaunchKernelGGL((k_ma_gather<Tg, 1>), grd, blk, 0, s, grid, gg, start, srec, ty, tz);
       break;
     case 3:
-      hipLaunchKernelGGL((k_ma_gather<Tg, 3>), grd, blk, 0, s, grid, gg, keys, srec, row_start,
-                         ty, tz);
+      hipLaunchKernelGGL((k_ma_gather<Tg, 3>), grd, blk, 0, s, grid, gg, start, srec, ty, tz);
       break;
   }
   FC_LAUNCH_CHECK();
@@ -276,14 +265,21 @@ void mass_assign(fc_grid* grid, const double* pos_dev, int64_t n, int ncol,
     }
   } evg{ev};
 
+  if (ncells >= 0x7fffffffULL) fail(FC_ERR_INVALID, "grid too large for mass assignment");
   const int nbp = (int)std::min<int64_t>(1024, (n + kPrepThreads - 1) / kPrepThreads);
-  DeviceBuffer keys(sizeof(uint64_t) * n), keys2(sizeof(uint64_t) * n);
-  DeviceBuffer idx(sizeof(uint32_t) * n), idx2(sizeof(uint32_t) * n);
-  DeviceBuffer rec(sizeof(double4) * n), srec(sizeof(double4) * n);
-  DeviceBuffer partw(sizeof(double) * 2 * (nbp + 1));
-  DeviceBuffer partn(sizeof(unsigned long long) * (nbp + 1));
-  const int64_t nrows = (int64_t)gg.n0 * gg.n1;
-  DeviceBuffer rows(sizeof(int64_t) * (nrows + 1));
+  // Work buffers live in the context and are reused by later calls.
+  DeviceBuffer* sc = ctx->scratch;
+  sc[1].reserve(sizeof(uint64_t) * n);
+  sc[2].reserve(sizeof(uint64_t) * n);
+  sc[3].reserve(sizeof(uint32_t) * n);
+  sc[4].reserve(sizeof(uint32_t) * n);
+  sc[5].reserve(sizeof(double4) * n);
+  sc[6].reserve(sizeof(double4) * n);
+  sc[7].reserve(sizeof(double) * 2 * (nbp + 1));
+  sc[8].reserve(sizeof(unsigned long long) * (nbp + 1));
+  sc[9].reserve(sizeof(int32_t) * (ncells + 1));
+  DeviceBuffer &keys = sc[1], &keys2 = sc[2], &idx = sc[3], &idx2 = sc[4], &rec = sc[5],
+               &srec = sc[6], &partw = sc[7], &partn = sc[8], &starts = sc[9];
 
   hipLaunchKernelGGL(k_ma_prepare, dim3(nbp), dim3(kPrepThreads), 0, s, pos_dev, n, ncol, w_dev,
                      w_scalar, gg, wrap, shift, sentinel, keys.as<uint64_t>(),
@@ -309,16 +305,17 @@ void mass_assign(fc_grid* grid, const double* pos_dev, int64_t n, int ncol,
   hipLaunchKernelGGL(k_permute, dim3(grid1d(n, 256, 65536)), dim3(256), 0, s,
                      idx2.as<uint32_t>(), rec.as<double4>(), srec.as<double4>(), n);
   FC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_row_starts, dim3(grid1d(nrows + 1, 256, 65536)), dim3(256), 0, s,
-                     keys2.as<uint64_t>(), n, nrows, gg.n2, rows.as<int64_t>());
+  hipLaunchKernelGGL(k_cell_starts, dim3(grid1d(n + 1, 256, 65536)), dim3(256), 0, s,
+                     keys2.as<uint64_t>(), partn.as<unsigned long long>() + nbp, ncells,
+                     starts.as<int32_t>());
   FC_LAUNCH_CHECK();
   FC_HIP(hipEventRecord(ev[1], s));
   if (grid->prec == FC_FP64)
-    launch_gather<double>(scheme, grid->data.as<double>(), gg, keys2.as<uint64_t>(),
-                          srec.as<double4>(), rows.as<int64_t>(), s);
+    launch_gather<double>(scheme, grid->data.as<double>(), gg, starts.as<int32_t>(),
+                          srec.as<double4>(), s);
   else
-    launch_gather<float>(scheme, grid->data.as<float>(), gg, keys2.as<uint64_t>(),
-                         srec.as<double4>(), rows.as<int64_t>(), s);
+    launch_gather<float>(scheme, grid->data.as<float>(), gg, starts.as<int32_t>(),
+                         srec.as<double4>(), s);
   FC_HIP(hipEventRecord(ev[2], s));
 
   double wsum[2];
