@@ -97,6 +97,19 @@ SIGNATURES = {
     "fc_corr_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fc_corr_set_algorithm": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fc_autotune": (ctypes.c_int, [_vp, _ip, ctypes.c_int, ctypes.c_int, _dp]),
+    "fc_apply_displacement": (ctypes.c_int, [_vp, _dp, _i64, _dp, ctypes.c_int, _dp]),
+    "fc_add_random_particles": (ctypes.c_int, [_vp, _i64, ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_int, _dp, ctypes.c_uint64, _i64,
+                                               ctypes.POINTER(MaStats)]),
+    "fc_random_uniform_device": (ctypes.c_int, [_vp, _vp, _i64, _dp, _dp, ctypes.c_uint64,
+                                                ctypes.c_uint64]),
+    "fc_random_raw_device": (ctypes.c_int, [_vp, _vp, _i64, ctypes.c_uint64, ctypes.c_uint64,
+                                            ctypes.c_uint32, ctypes.c_uint32]),
+    "fc_random_uniform": (ctypes.c_int, [_vp, _dp, _i64, _dp, _dp, ctypes.c_uint64,
+                                         ctypes.c_uint64]),
+    "fc_random_raw": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32), _i64,
+                                     ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                     ctypes.c_uint32]),
     "fc_batch_create": (ctypes.c_int, [_vp, _ip, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int,

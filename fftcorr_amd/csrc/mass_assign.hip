@@ -339,6 +339,13 @@ void mass_assign(fc_grid* grid, const double* pos_dev, int64_t n, int ncol,
 }
 
 }  // namespace
+
+void mass_assign_device(fc_grid* grid, const double* pos_dev, int64_t n, int ncol,
+                        const double* w_dev, double w_scalar, int scheme, int wrap, double shift,
+                        fc_ma_stats* stats) {
+  mass_assign(grid, pos_dev, n, ncol, w_dev, w_scalar, scheme, wrap, shift, stats);
+}
+
 }  // namespace fc
 
 extern "C" {

@@ -114,6 +114,33 @@ int fc_mass_assign_device(fc_grid* grid, const double* pos_dev, int64_t n, int n
                           const double* weights_dev, double w_scalar, int scheme,
                           int periodic_wrap, double shift, fc_ma_stats* stats);
 
+/* ---- Randoms and displacement fields: fftcorr/catalog/random_catalog.py:8-87,
+ *      fftcorr/grid/utils.pyx:19-55 ------------------------------------------ */
+/* apply_displacement_field: out[i] = pos[i] + disp[cell(pos[i])] for host
+ * arrays pos/out [n][3] and disp [n0][n1][n2][3]; FC_ERR_INVALID (the
+ * reference's ValueError) if a position is outside the grid without wrap. */
+int fc_apply_displacement(fc_grid* grid, const double* pos, int64_t n, const double* disp,
+                          int periodic_wrap, double* out);
+/* add_random_particles with the randoms drawn on the device: n uniform points
+ * in the grid's box from the counter-based stream `seed` (Philox4x32-10),
+ * optionally displaced by disp (host [n0][n1][n2][3] or NULL), assigned with
+ * weight `weight` in batches of `batch`, statistics accumulated into stats. */
+int fc_add_random_particles(fc_grid* grid, int64_t n, double weight, int scheme,
+                            int periodic_wrap, const double* disp, uint64_t seed, int64_t batch,
+                            fc_ma_stats* stats);
+/* Device pointers: pos [n][3] = particles first .. first+n-1 of stream seed,
+ * uniform in [lo, hi) per axis; raw: out [n][4] Philox4x32-10 words of the
+ * counters (first + i, w2, w3) under key seed (known-answer testing). */
+int fc_random_uniform_device(fc_ctx* ctx, double* pos, int64_t n, const double lo[3],
+                             const double hi[3], uint64_t seed, uint64_t first);
+int fc_random_raw_device(fc_ctx* ctx, uint32_t* out, int64_t n, uint64_t seed, uint64_t first,
+                         uint32_t w2, uint32_t w3);
+/* The same into host arrays. */
+int fc_random_uniform(fc_ctx* ctx, double* pos, int64_t n, const double lo[3], const double hi[3],
+                      uint64_t seed, uint64_t first);
+int fc_random_raw(fc_ctx* ctx, uint32_t* out, int64_t n, uint64_t seed, uint64_t first,
+                  uint32_t w2, uint32_t w3);
+
 /* ---- Correlators: cc/correlate/correlator.h:18-505,
  *      fftcorr/correlate/correlator.pyx:12-184 ----------------------------- */
 /* kind FC_PERIODIC or FC_YAMAMOTO; posmin is used by FC_YAMAMOTO only (observer

@@ -272,6 +272,20 @@ def grid_coords(pos, posmin, cell_size, shape, periodic_wrap):
 
 
 # ---------------------------------------------------------------------------
+# fftcorr/grid/utils.pyx:19-55 (apply_displacement_field), vectorised.
+def apply_displacement_field(pos, disp, posmin, cell_size, shape, periodic_wrap):
+    """out[i] = pos[i] + disp[int(gx), int(gy), int(gz)] with (gx, gy, gz)
+    the grid coordinates of pos[i]; ValueError if any is outside the grid
+    without wrap (the reference stops at the first such particle)."""
+    x, ok = grid_coords(pos, posmin, cell_size, shape, periodic_wrap)
+    if not ok.all():
+        raise ValueError("Failed to apply displacement field: coordinates out of bounds? "
+                         "Try setting periodic_wrap=True.")
+    ic = x.astype(np.int64)
+    return np.asarray(pos, np.float64) + disp[ic[:, 0], ic[:, 1], ic[:, 2]]
+
+
+# ---------------------------------------------------------------------------
 # Mass assignment.  cc/particle_mesh/distribution_functions.h:25-160 and
 # cc/particle_mesh/mass_assignor.h:105-191.
 def _stencil_1d(scheme, r):
