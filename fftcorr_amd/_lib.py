@@ -97,6 +97,7 @@ SIGNATURES = {
     "fc_corr_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fc_corr_set_algorithm": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fc_autotune": (ctypes.c_int, [_vp, _ip, ctypes.c_int, ctypes.c_int, _dp]),
+    "fc_set_wavelet_table": (ctypes.c_int, [_dp, ctypes.c_int]),
     "fc_apply_displacement": (ctypes.c_int, [_vp, _dp, _i64, _dp, ctypes.c_int, _dp]),
     "fc_add_random_particles": (ctypes.c_int, [_vp, _i64, ctypes.c_double, ctypes.c_int,
                                                ctypes.c_int, _dp, ctypes.c_uint64, _i64,

@@ -7,8 +7,9 @@
 Options: -n/-ngrid, -n3/-ngrid3, -ell/-maxell, -r/-sep, -dr/-dsep, -kmax/-k,
 -dk, -cell/-c, -in/-i, -in2/-i2, -out/-o, -periodic/-p, -dist/-d, -wisdom/-wf,
 plus -precision fp64|fp32|mixed (default fp64).  Distribution scheme 2 (the
-D12 wavelet, the reference's default) is not available on the GPU: pass -d 0
-(NGP), 1 (TSC) or 3 (CIC).  A TSC run uses the TSC-aliased window correction
+D12 wavelet, the reference's default) needs the wavelet's sample table
+registered first (fftcorr_amd.particle_mesh.set_wavelet_table); otherwise pass
+-d 0 (NGP), 1 (TSC) or 3 (CIC).  A TSC run uses the TSC-aliased window correction
 and shot-noise factor, like the reference; CIC uses the CIC-aliased window.
 """
 
@@ -111,9 +112,12 @@ def hist_lines(bins, counts, hist, prefix, normalize):
 
 def run(argv, stdout=None):
     o = parse(argv)
-    if o["scheme"] not in (0, 1, 3):
-        raise SystemExit("distribution scheme %d is not available on the GPU "
-                         "(use -d 0 NGP, 1 TSC or 3 CIC)" % o["scheme"])
+    from fftcorr_amd.particle_mesh import wavelet_table_set
+    if o["scheme"] not in (0, 1, 2, 3) or (o["scheme"] == 2 and not wavelet_table_set()):
+        raise SystemExit("distribution scheme %d is not available on the GPU (use -d 0 NGP, "
+                         "1 TSC or 3 CIC; -d 2, the D12 wavelet, needs its sample table "
+                         "registered with fftcorr_amd.particle_mesh.set_wavelet_table)"
+                         % o["scheme"])
     t0 = time.perf_counter()
     w = []
     w.append("# Running on the GPU (fftcorr_amd).")
@@ -157,6 +161,8 @@ def run(argv, stdout=None):
     elif o["scheme"] == 1:
         totwsq *= 0.55 * 0.55 * 0.55
         w.append("# Using triangular cloud-in-cell")
+    elif o["scheme"] == 2:
+        w.append("# Using D12 wavelet")
     else:
         w.append("# Using cloud-in-cell")
     vcell = cell ** 3

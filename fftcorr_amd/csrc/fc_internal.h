@@ -127,6 +127,8 @@ struct fc_ctx {
   hipStream_t stream = nullptr;
   // Scratch reused across calls on this context (sort temp storage, staging).
   fc::DeviceBuffer scratch[12];  // per-context work buffers, grown on demand
+  fc::DeviceBuffer wavelet;       // D12 sample table on this device
+  uint64_t wavelet_ver = 0;       // version of the process-wide table it holds
   int arch_cus = 0;
 };
 

@@ -17,6 +17,9 @@
 // No floating-point atomics: results do not depend on scheduling.
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+#include <vector>
+
 #include "fc_objects.h"
 
 namespace fc {
@@ -209,9 +212,89 @@ __global__ void __launch_bounds__(TX* TY* TZ)
   }
 }
 
+// D12 wavelet (distribution_functions.h:162-217): a particle in cell s with
+// sub-cell sample (sx, sy, sz) = floor(64 r) adds
+// tab[ox][sx] w * tab[oy][sy] * tab[oz][sz] to cell s - 1 + o, o in [0, 8)^3,
+// tab being the caller's 8 x 64 sample table (d12.h's wavesample order).
+// Target t therefore gathers from source cells t - 6 .. t + 1 per axis.
+constexpr int WC = 8, WS = 64;
+constexpr int DX = TX + WC - 1, DY = TY + WC - 1, DZ = TZ + WC - 1;
+
+template <typename Tg>
+__global__ void __launch_bounds__(TX* TY* TZ)
+    k_ma_gather_wavelet(Tg* __restrict__ grid, GridGeom g, const int32_t* __restrict__ start,
+                        const double4* __restrict__ srec, const double* __restrict__ table,
+                        int tiles_y, int tiles_z) {
+#pragma clang fp contract(off)
+  __shared__ int32_t cs[DX * DY * DZ];
+  __shared__ int32_t ce[DX * DY * DZ];
+  __shared__ double tab[WC * WS];
+  int b = blockIdx.x;
+  const int bz = b % tiles_z;
+  b /= tiles_z;
+  const int by = b % tiles_y;
+  const int bx = b / tiles_y;
+  const int x0 = bx * TX, y0 = by * TY, z0 = bz * TZ;
+  for (int i = threadIdx.x; i < WC * WS; i += blockDim.x) tab[i] = table[i];
+  for (int h = threadIdx.x; h < DX * DY * DZ; h += blockDim.x) {
+    const int hz = h % DZ, hy = (h / DZ) % DY, hx = h / (DZ * DY);
+    const int gx = ((x0 + hx - (WC - 2)) % g.n0 + g.n0) % g.n0;
+    const int gy = ((y0 + hy - (WC - 2)) % g.n1 + g.n1) % g.n1;
+    const int gz = ((z0 + hz - (WC - 2)) % g.n2 + g.n2) % g.n2;
+    const int64_t c = ((int64_t)gx * g.n1 + gy) * g.n2 + gz;
+    cs[h] = start[c];
+    ce[h] = start[c + 1];
+  }
+  __syncthreads();
+  const int tz = threadIdx.x % TZ, ty = (threadIdx.x / TZ) % TY, tx = threadIdx.x / (TZ * TY);
+  const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
+  if (x >= g.n0 || y >= g.n1 || z >= g.n2) return;
+  double sum = 0.0;
+  bool touched = false;
+  for (int ox = 0; ox < WC; ++ox)
+    for (int oy = 0; oy < WC; ++oy)
+      for (int oz = 0; oz < WC; ++oz) {
+        const int h = ((tx + WC - 1 - ox) * DY + (ty + WC - 1 - oy)) * DZ + (tz + WC - 1 - oz);
+        touched |= cs[h] < ce[h];
+        for (int p = cs[h]; p < ce[h]; ++p) {
+          const double4 q = srec[p];
+          const int sx = (int)floor(q.x * WS), sy = (int)floor(q.y * WS), sz = (int)floor(q.z * WS);
+          const double dxw = tab[ox * WS + sx] * q.w;
+          const double dxy = dxw * tab[oy * WS + sy];
+          sum += tab[oz * WS + sz] * dxy;
+        }
+      }
+  if (touched) {
+    Tg* cell = grid + ((int64_t)x * g.n1 + y) * g.n2 + z;
+    *cell = (Tg)((double)*cell + sum);
+  }
+}
+
+// Process-wide D12 table (fc_set_wavelet_table), copied to each context's
+// device on first use after a change.
+std::mutex g_wavelet_mu;
+std::vector<double> g_wavelet;
+uint64_t g_wavelet_ver = 0;
+
+const double* wavelet_on_device(fc_ctx* ctx) {
+  std::lock_guard<std::mutex> lk(g_wavelet_mu);
+  if (g_wavelet.empty())
+    fail(FC_ERR_INVALID,
+         "distribution scheme 2 (D12 wavelet) needs its sample table: call "
+         "fc_set_wavelet_table first");
+  if (ctx->wavelet_ver != g_wavelet_ver) {
+    ctx->wavelet.reserve(sizeof(double) * g_wavelet.size());
+    FC_HIP(hipMemcpyAsync(ctx->wavelet.as<void>(), g_wavelet.data(),
+                          sizeof(double) * g_wavelet.size(), hipMemcpyHostToDevice, ctx->stream));
+    FC_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->wavelet_ver = g_wavelet_ver;
+  }
+  return ctx->wavelet.as<double>();
+}
+
 template <typename Tg>
 void launch_gather(int scheme, Tg* grid, const GridGeom& gg, const int32_t* start,
-                   const double4* srec, hipStream_t s) {
+                   const double4* srec, hipStream_t s, const double* table = nullptr) {
   const int tx = (gg.n0 + TX - 1) / TX, ty = (gg.n1 + TY - 1) / TY, tz = (gg.n2 + TZ - 1) / TZ;
   const uint64_t nb = (uint64_t)tx * ty * tz;
   if (nb > 0x7fffffffULL) fail(FC_ERR_INVALID, "grid too large for mass assignment");
@@ -226,6 +309,10 @@ void launch_gather(int scheme, Tg* grid, const GridGeom& gg, const int32_t* star
     case 3:
       hipLaunchKernelGGL((k_ma_gather<Tg, 3>), grd, blk, 0, s, grid, gg, start, srec, ty, tz);
       break;
+    case 2:
+      hipLaunchKernelGGL((k_ma_gather_wavelet<Tg>), grd, blk, 0, s, grid, gg, start, srec,
+                         table, ty, tz);
+      break;
   }
   FC_LAUNCH_CHECK();
 }
@@ -239,9 +326,7 @@ int key_bits(uint64_t sentinel) {
 void mass_assign(fc_grid* grid, const double* pos_dev, int64_t n, int ncol,
                  const double* w_dev, double w_scalar, int scheme, int wrap, double shift,
                  fc_ma_stats* stats) {
-  if (scheme == 2)
-    fail(FC_ERR_INVALID, "distribution scheme 2 (D12 wavelet) is not implemented on the GPU");
-  if (scheme != 0 && scheme != 1 && scheme != 3)
+  if (scheme < 0 || scheme > 3)
     fail(FC_ERR_INVALID, "unknown distribution scheme " + std::to_string(scheme));
   if (ncol != 3 && ncol != 4) fail(FC_ERR_INVALID, "Rows must be 3D or 4D.");
   if (n < 0) fail(FC_ERR_INVALID, "negative particle count");
@@ -254,6 +339,11 @@ void mass_assign(fc_grid* grid, const double* pos_dev, int64_t n, int ncol,
   const uint64_t sentinel = ncells;
   GridGeom gg{grid->shape[0], grid->shape[1], grid->shape[2], grid->posmin[0],
               grid->posmin[1], grid->posmin[2], grid->cell};
+  const double* wtab = scheme == 2 ? wavelet_on_device(ctx) : nullptr;
+  if (scheme == 2)
+    for (int d = 0; d < 3; ++d)
+      if (grid->shape[d] < WC)
+        fail(FC_ERR_INVALID, "the D12 wavelet needs at least 8 cells per axis");
   if (n == 0) return;
 
   hipEvent_t ev[3];
@@ -312,10 +402,10 @@ void mass_assign(fc_grid* grid, const double* pos_dev, int64_t n, int ncol,
   FC_HIP(hipEventRecord(ev[1], s));
   if (grid->prec == FC_FP64)
     launch_gather<double>(scheme, grid->data.as<double>(), gg, starts.as<int32_t>(),
-                          srec.as<double4>(), s);
+                          srec.as<double4>(), s, wtab);
   else
     launch_gather<float>(scheme, grid->data.as<float>(), gg, starts.as<int32_t>(),
-                         srec.as<double4>(), s);
+                         srec.as<double4>(), s, wtab);
   FC_HIP(hipEventRecord(ev[2], s));
 
   double wsum[2];
@@ -349,6 +439,19 @@ void mass_assign_device(fc_grid* grid, const double* pos_dev, int64_t n, int nco
 }  // namespace fc
 
 extern "C" {
+
+int fc_set_wavelet_table(const double* table, int n) {
+  return fc::guard([&] {
+    if (table && n != fc::WC * fc::WS)
+      fc::fail(FC_ERR_INVALID, "the D12 table has 8 x 64 = 512 samples");
+    std::lock_guard<std::mutex> lk(fc::g_wavelet_mu);
+    if (table)
+      fc::g_wavelet.assign(table, table + n);
+    else
+      fc::g_wavelet.clear();
+    ++fc::g_wavelet_ver;
+  });
+}
 
 int fc_mass_assign_device(fc_grid* grid, const double* pos_dev, int64_t n, int ncol,
                           const double* weights_dev, double w_scalar, int scheme,

@@ -15,6 +15,29 @@ from fftcorr_amd import _lib
 from fftcorr_amd._arrays import check_row_major
 
 
+_WAVELET = {"set": False}
+
+
+def set_wavelet_table(table):
+    """Registers the D12 wavelet's sample table (cc/particle_mesh/d12.h,
+    `wavesample`: 8 cells x 64 sub-cell samples, 512 values in that order)
+    for distribution scheme 2; None clears it.  The table is the reference's
+    data and is not shipped with this package."""
+    if table is None:
+        _lib.check(_lib.lib().fc_set_wavelet_table(None, 0))
+        _WAVELET["set"] = False
+        return
+    t = np.ascontiguousarray(table, np.float64).reshape(-1)
+    if t.size != 512:
+        raise ValueError("the D12 table has 8 x 64 = 512 samples")
+    _lib.check(_lib.lib().fc_set_wavelet_table(_lib.dptr(t), 512))
+    _WAVELET["set"] = True
+
+
+def wavelet_table_set():
+    return _WAVELET["set"]
+
+
 class MassAssignor:
     def __init__(self, grid, distribution_scheme, periodic_wrap=False, buffer_size=0,
                  shift=0.0):
@@ -22,9 +45,12 @@ class MassAssignor:
         if not isinstance(grid, ConfigSpaceGrid):
             raise TypeError("grid must be a ConfigSpaceGrid")
         distribution_scheme = int(distribution_scheme)
-        if distribution_scheme not in (0, 1, 3):
-            raise ValueError("distribution scheme %d is not supported on the GPU "
-                             "(0 NGP, 1 TSC, 3 CIC)" % distribution_scheme)
+        if distribution_scheme not in (0, 1, 2, 3):
+            raise ValueError("unknown distribution scheme %d (0 NGP, 1 TSC, 2 D12 wavelet, "
+                             "3 CIC)" % distribution_scheme)
+        if distribution_scheme == 2 and not wavelet_table_set():
+            raise ValueError("distribution scheme 2 (D12 wavelet) needs its sample table: "
+                             "call fftcorr_amd.particle_mesh.set_wavelet_table first")
         if ("distribution_scheme" in grid.metadata
                 and grid.metadata["distribution_scheme"] != distribution_scheme):
             grid.metadata["distribution_scheme"] = -1

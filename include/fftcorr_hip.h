@@ -106,6 +106,11 @@ typedef struct fc_ma_stats {
  * w_scalar is every particle's weight).  shift (grid units) displaces every
  * particle before assignment (0.5 builds an interlaced grid; new).  Stats are
  * accumulated into *stats.  pos/weights are host pointers. */
+/* Scheme 2 (D12 wavelet, distribution_functions.h:162-217) uses the sample
+ * table of cc/particle_mesh/d12.h (8 cells x 64 samples, `wavesample`
+ * order), which the caller supplies (it is the reference's data, not part of
+ * this library); NULL clears it.  Process-wide. */
+int fc_set_wavelet_table(const double* table, int n);
 int fc_mass_assign(fc_grid* grid, const double* pos, int64_t n, int ncol,
                    const double* weights, double w_scalar, int scheme,
                    int periodic_wrap, double shift, fc_ma_stats* stats);

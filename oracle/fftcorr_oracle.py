@@ -16,6 +16,8 @@ Pins (what checks each function; see tests/test_oracle_*.py):
   window functions     closed forms + bit-exact against oracle/_ref
   grid coords / skip   test_grid.py known answers + oracle/_ref
   NGP / TSC            oracle/_ref (distribution_functions.h) + sum(grid)=sum(w)
+  D12 wavelet          oracle/_ref (its own table and stencil) + partition of
+                       unity of the table columns + sum(grid)=sum(w)
   CIC (new, no ref)    closed form (trilinear about cell centres) + mass
                        conservation + linear-field reproduction
   FFT conventions      brute-force DFT on tiny grids, Parseval
@@ -288,8 +290,15 @@ def apply_displacement_field(pos, disp, posmin, cell_size, shape, periodic_wrap)
 # ---------------------------------------------------------------------------
 # Mass assignment.  cc/particle_mesh/distribution_functions.h:25-160 and
 # cc/particle_mesh/mass_assignor.h:105-191.
-def _stencil_1d(scheme, r):
-    """(offsets, weights[n, len(offsets)]) along one axis, r = x - floor(x)."""
+def _stencil_1d(scheme, r, table=None):
+    """(offsets, weights[n, len(offsets)]) along one axis, r = x - floor(x).
+    ``table`` is the D12 sample table (8 x 64, d12.h order) for WAVELET."""
+    if scheme == WAVELET:  # distribution_functions.h:176-216, d12.h:14-16
+        if table is None:
+            raise ValueError("the D12 wavelet needs its sample table")
+        tab = np.asarray(table, np.float64).reshape(8, 64)
+        sub = np.floor(r * 64).astype(np.int64)
+        return list(range(-1, 7)), tab[:, sub].T
     if scheme == NEAREST_CELL:
         return [0], np.ones((len(r), 1))
     if scheme == TRIANGULAR_SHAPED_CLOUD:  # :59-67
@@ -309,10 +318,10 @@ def _stencil_1d(scheme, r):
     raise ValueError("Unsupported distribution scheme %r" % (scheme,))
 
 
-def assign_mass(grid, coords, weights, scheme):
+def assign_mass(grid, coords, weights, scheme, table=None):
     """Adds the particles (grid coords, already wrapped/filtered) to grid
     (float64 [n0,n1,n2]) in place.  The stencil always wraps periodically, as
-    the reference's TSC does (distribution_functions.h:69-119)."""
+    the reference's TSC and D12 do (distribution_functions.h:69-119, 189-214)."""
     n0, n1, n2 = grid.shape
     coords = np.asarray(coords, np.float64)
     w = np.asarray(weights, np.float64)
@@ -321,7 +330,7 @@ def assign_mass(grid, coords, weights, scheme):
     offs = []
     wts = []
     for d in range(3):
-        o, wd = _stencil_1d(scheme, r[:, d])
+        o, wd = _stencil_1d(scheme, r[:, d], table)
         offs.append(o)
         wts.append(wd)
     flat = grid.reshape(-1)

@@ -96,6 +96,17 @@ def grid_coords(shape, posmin, cell, wrap, pos):
     return bool(ok), out
 
 
+def wavelet_table():
+    """The reference's D12 sample table (512 values, 8 cells x 64 samples)."""
+    out = np.zeros(512)
+    lib = load()
+    lib.ref_wavelet_table.argtypes = [_dp]
+    lib.ref_wavelet_table.restype = ctypes.c_int
+    if lib.ref_wavelet_table(_ptr(out)) != 512:
+        raise RuntimeError("unexpected wavelet table size")
+    return out
+
+
 def mass_assign(shape, posmin, cell, scheme, wrap, posw, buffer_size=0):
     s = np.array(shape, np.intc)
     pm = np.array(posmin, np.float64)

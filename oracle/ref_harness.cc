@@ -167,4 +167,11 @@ int ref_cli_grid(const char* file1, const char* file2, int ngrid, double sep,
   return 0;
 }
 
+// The D12 wavelet's sample table in the reference's own order
+// (d12.h: 8 cells x 64 sub-cell samples).
+int ref_wavelet_table(double* out) {
+  for (int i = 0; i < WCELLS * WAVESAMPLE; ++i) out[i] = wavesample[i];
+  return WCELLS * WAVESAMPLE;
+}
+
 }  // extern "C"

@@ -95,3 +95,33 @@ def test_cic_closed_form_and_invariants():
     for d in range(3):
         marg = g1.sum(axis=tuple(i for i in range(3) if i != d))
         assert np.dot(marg, centres[d]) == pytest.approx(p[0, d], rel=1e-14)
+
+
+def test_d12_table_invariants(ref):
+    # Each sub-cell sample's 8 cell weights sum to one (mass conservation),
+    # and the weights' first moment over the offsets -1..6 moves by one
+    # sample (1/64 cell) per sample step, to within 1%.
+    tab = ref.wavelet_table().reshape(8, 64)
+    np.testing.assert_allclose(tab.sum(axis=0), 1.0, atol=1e-9)
+    step = np.diff(np.arange(-1, 7) @ tab)
+    np.testing.assert_allclose(np.abs(step), 1 / 64, rtol=0.01)
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (9, 12, 10), (16, 16, 16)])
+@pytest.mark.parametrize("wrap", [False, True])
+def test_d12_against_reference(ref, shape, wrap):
+    rng = np.random.default_rng(sum(shape) * 3 + wrap)
+    n = 400
+    posmin = np.array([-10.0, 3.0, 100.0])
+    cell = 7.0
+    lo = posmin - (cell * 2 if wrap else 0)
+    hi = posmin + cell * np.array(shape) + (cell * 2 if wrap else 0)
+    pos = rng.uniform(lo, hi, size=(n, 3))
+    posw = np.concatenate([pos, rng.uniform(-1.5, 1.5, (n, 1))], axis=1)
+    gref, st = ref.mass_assign(shape, posmin, cell, O.WAVELET, wrap, posw, buffer_size=64)
+    x, ok = O.grid_coords(pos, posmin, cell, shape, wrap)
+    tab = ref.wavelet_table()
+    g = O.assign_mass(np.zeros(shape), x[ok], posw[ok, 3], O.WAVELET, tab)
+    np.testing.assert_allclose(g, gref, rtol=1e-12, atol=1e-13)
+    assert g.sum() == pytest.approx(posw[ok, 3].sum(), abs=1e-9)
+    assert st["num_added"] == ok.sum()
