@@ -11,8 +11,10 @@ inverse FFT and the r-space multipole histograms, and leaves FP64 histograms
 on the device.
 
 Forms timed on the same grids:
-  * mixed, fused (the headline `value`): FP32 grids, forward FFTs and
-    products, FP64 pruned inverse FFT; FP64 Y_lm, windows, bins and sums;
+  * mixed, fused (the headline `value`): FP32 grids and spectra in HBM;
+    the z and y forward FFT passes compute in FP64 (rounding to FP32 at
+    each LDS exchange and store), the x pass and products in FP32; FP64
+    pruned inverse FFT, Y_lm, windows, bins and sums;
   * fp32, fused: the inverse in FP32 as well;
   * fp64, fused: every stage in FP64;
   * fp64, plain: the reference's stage sequence (full forward FFT,
@@ -467,8 +469,9 @@ def main():
                         "rmax 200 dr 5",
             "grid": cfg["n"], "maxell": cfg["maxell"], "particles": int(pos.shape[0]),
             "k_subgrid": list(fm["kshape"]), "r_subgrid": list(fm["rshape"]),
-            "precision": "mixed: FP32 grids, forward FFTs and products; FP64 pruned inverse "
-                         "FFT, Y_lm, windows, bins and sums",
+            "precision": "mixed: FP32 grids and spectra in HBM and LDS; FP64 arithmetic in "
+                         "the z and y forward FFT passes; FP32 x pass and products; FP64 "
+                         "pruned inverse FFT, Y_lm, windows, bins and sums",
         },
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),

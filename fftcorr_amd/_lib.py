@@ -173,7 +173,8 @@ def iptr(a):
 
 def precision_code(precision):
     """Correlator precision: "fp64" (FC_FP64), "fp32" (FC_FP32) or "mixed"
-    (FC_MIXED: FP32 grids, forward FFTs and products, FP64 inverse)."""
+    (FC_MIXED: FP32 grids and spectra, FP64 arithmetic in the z and y forward
+    FFT passes, FP64 inverse)."""
     if isinstance(precision, str):
         key = precision.lower()
     else:

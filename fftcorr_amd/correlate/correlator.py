@@ -6,7 +6,9 @@ Same constructors, ``from_grid_spec``, ``autocorrelate``, ``cross_correlate``,
 (power_spectrum_table, correlation_table).  Additions:
   * ``precision``: "fp64" (default; every stage in FP64 like the reference),
     "fp32" (FP32 grids/FFTs/products, FP64 sums and histograms) or "mixed"
-    ("fp32" with the pruned inverse FFT in FP64);
+    (FP32 storage as "fp32"; the z and y forward FFT passes compute in FP64
+    and round to FP32 only where values go to LDS or HBM; the pruned
+    inverse FFT in FP64);
   * ``algorithm``: "fused" (default; fused x pass, pruned inverse) or
     "plain" (the reference's stage sequence on the device, fp64/fp32 only);
   * ``device``: the GPU to run on; ``context``: an explicit fftcorr_amd

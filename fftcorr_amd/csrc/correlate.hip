@@ -556,12 +556,15 @@ void check_grid_shape(fc_corr* c, fc_grid* g) {
     fail(FC_ERR_INVALID, "grid and correlator live on different devices");
 }
 
-// z and y forward passes of a device grid into dst (complex rows of ldc).
-// The grid is processed in chunks of x slabs (plan zy_chunk) so that each
-// chunk's y pass reads the z pass's output from the last-level cache.
-template <typename T, class Loader>
+// z and y forward passes of a device grid into dst (complex rows of ldc,
+// stored in T; arithmetic in A, which is wider than T for the mixed form and
+// then takes its twiddles from the FP64 plan plan_inv).  The grid is
+// processed in chunks of x slabs (plan zy_chunk) so that each chunk's y pass
+// reads the z pass's output from the last-level cache.
+template <typename T, typename A = T, class Loader>
 void forward_zy(fc_corr* c, fc_grid* g, cplx<T>* dst, Loader ld) {
   const FftPlan& p = *c->plan;
+  const FftPlan& pa = *c->plan_inv;
   hipStream_t s = c->ctx->stream;
   const int X = p.zy_chunk > 0 ? p.zy_chunk : p.n[0];
   for (int x0 = 0; x0 < p.n[0]; x0 += X) {
@@ -574,29 +577,34 @@ void forward_zy(fc_corr* c, fc_grid* g, cplx<T>* dst, Loader ld) {
       Stage st(c, FC_STAGE_FFT_Z_FWD);
       const int64_t in0 = (int64_t)x0 * p.n[1] * p.n[2];
       if (g->prec == FC_FP64)
-        launch_rows_r2c<T, double>(p, g->data.as<double>() + in0, p.n[2], d, p.ldc, lc, s, rows);
+        launch_rows_r2c<T, double, Loader, A>(p, g->data.as<double>() + in0, p.n[2], d, p.ldc,
+                                              lc, s, rows, &pa);
       else
-        launch_rows_r2c<T, float>(p, g->data.as<float>() + in0, p.n[2], d, p.ldc, lc, s, rows);
+        launch_rows_r2c<T, float, Loader, A>(p, g->data.as<float>() + in0, p.n[2], d, p.ldc, lc,
+                                             s, rows, &pa);
     }
     Stage st(c, FC_STAGE_FFT_Y_FWD);
     ColView<T> v = y_view<T>(p, d);
     v.A = nx;
-    launch_cols<T, -1>(p.n[1], v, p.tile[1], p.tw1.as<cplx<T>>(), s);
+    launch_cols_wide<T, -1, A>(p.n[1], v, p.tile[1], pa.tw1.as<cplx<A>>(), p.tw1.as<cplx<T>>(),
+                               s);
   }
 }
 
-template <typename T>
+template <typename T, typename A = T>
 void forward_x(fc_corr* c, cplx<T>* dst) {
   const FftPlan& p = *c->plan;
   Stage st(c, FC_STAGE_FFT_X_FWD);
-  launch_cols<T, -1>(p.n[0], x_view<T>(p, dst), p.tile[0], p.tw0.as<cplx<T>>(), c->ctx->stream);
+  launch_cols_wide<T, -1, A>(p.n[0], x_view<T>(p, dst), p.tile[0],
+                             c->plan_inv->tw0.as<cplx<A>>(), p.tw0.as<cplx<T>>(),
+                             c->ctx->stream);
 }
 
 // Full forward 3D FFT.
-template <typename T, class Loader>
+template <typename T, typename A = T, class Loader>
 void forward_fft(fc_corr* c, fc_grid* g, cplx<T>* dst, Loader ld) {
-  forward_zy<T>(c, g, dst, ld);
-  forward_x<T>(c, dst);
+  forward_zy<T, A>(c, g, dst, ld);
+  forward_x<T, A>(c, dst);
 }
 
 // exp(i pi f / n) for the signed frequency f of every index along each axis
@@ -739,13 +747,13 @@ void fused_x(fc_corr* c, cplx<T>* w, const cplx<T>* b) {
 
 // Forward x pass(es), interlacing and the conjugate product over the whole
 // work grid w (the reference's execute_fft + multiply_with_conjugation).
-template <typename T>
+template <typename T, typename A = T>
 void full_power(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode) {
   hipStream_t s = c->ctx->stream;
   const FftPlan& p = *c->plan;
   const int64_t nc = p.nrows() * p.ldc;
-  forward_x<T>(c, w);
-  if (mode == kModeInterlaced) forward_x<T>(c, const_cast<cplx<T>*>(b));
+  forward_x<T, A>(c, w);
+  if (mode == kModeInterlaced) forward_x<T, A>(c, const_cast<cplx<T>*>(b));
   Stage st(c, FC_STAGE_COMBINE);
   if (mode == kModeInterlaced) {
     hipLaunchKernelGGL(k_interlace<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, w, b,
@@ -791,7 +799,7 @@ void power_and_pruned_inverse(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode
   } else {
     // Other x lengths: separate x pass, combine, gather, pruned inverse x
     // (reading the power grid in T, transforming in TI).
-    full_power<T>(c, w, b, mode);
+    full_power<T, TI>(c, w, b, mode);
     gather_pk<T>(c, w);
     Stage st(c, FC_STAGE_FFT_X_INV);
     ColView<TI, T> v{w, 0, (int64_t)p.n[1] * p.ldc, (int64_t)p.ldc, c->q1.as<cplx<TI>>(), 0,
@@ -864,11 +872,11 @@ void periodic_correlate(fc_corr* c, fc_grid* g1, int mode, fc_grid* gb) {
   cplx<T>* w = c->work.as<cplx<T>>();
   ensure_periodic_buffers<T, TI>(c);
   ensure_phase_tables(c);
-  forward_zy<T>(c, g1, w, LoadCopy{});
+  forward_zy<T, TI>(c, g1, w, LoadCopy{});
   const cplx<T>* b = nullptr;
   if (mode == kModeInterlaced) {
     c->work_b.reserve(p.complex_bytes());
-    forward_zy<T>(c, gb, c->work_b.as<cplx<T>>(), LoadCopy{});
+    forward_zy<T, TI>(c, gb, c->work_b.as<cplx<T>>(), LoadCopy{});
     b = c->work_b.as<cplx<T>>();
   } else if (mode == kModeCross) {
     b = c->grid2_fft.as<cplx<T>>();
@@ -1039,7 +1047,7 @@ void yamamoto_correlate(fc_corr* c, fc_grid* g1) {
   for (int ell = 0; ell <= c->maxell; ell += 2) {
     for (int m = -ell; m <= ell; ++m) {
       LoadYlm ld{ell, m, p.n[1], ylm_coeff(ell, m, 1.0), start[0], start[1], start[2]};
-      forward_zy<T>(c, g1, w, ld);
+      forward_zy<T, TI>(c, g1, w, ld);
       power_and_pruned_inverse<T, TI>(c, w, c->grid2_fft.as<cplx<T>>(), kModeCross);
       plain_k_multipole<T>(c, ell, m, 4.0 * M_PI, m != -ell);
       Stage st(c, FC_STAGE_SUBGRID);
@@ -1060,6 +1068,8 @@ void set_grid2_device(fc_corr* c, fc_grid* g2) {
   c->grid2_fft.reserve(c->plan->complex_bytes());
   if (c->prec == FC_FP64)
     forward_fft<double>(c, g2, c->grid2_fft.as<cplx<double>>(), LoadCopy{});
+  else if (c->prec == FC_MIXED && c->algo != FC_ALGO_PLAIN)
+    forward_fft<float, double>(c, g2, c->grid2_fft.as<cplx<float>>(), LoadCopy{});
   else
     forward_fft<float>(c, g2, c->grid2_fft.as<cplx<float>>(), LoadCopy{});
   c->have_grid2 = true;
@@ -1374,14 +1384,19 @@ void batch_sequence(fc_batch* bt, int n) {
   fill<double>(bt->zerolag.as<double>(), n, 0.0, s);
   cplx<T>* w = bt->work.as<cplx<T>>();
   const int64_t rows = (int64_t)n * p.n[0] * p.n[1];
+  // Forward z and y in the arithmetic of TI (FP64 for the mixed form),
+  // stored in T.
   if (bt->grid_prec == FC_FP64)
-    launch_rows_r2c<T, double>(p, bt->input.as<double>(), p.n[2], w, p.ldc, LoadCopy{}, s, rows);
+    launch_rows_r2c<T, double, LoadCopy, TI>(p, bt->input.as<double>(), p.n[2], w, p.ldc,
+                                             LoadCopy{}, s, rows, &pi);
   else
-    launch_rows_r2c<T, float>(p, bt->input.as<float>(), p.n[2], w, p.ldc, LoadCopy{}, s, rows);
+    launch_rows_r2c<T, float, LoadCopy, TI>(p, bt->input.as<float>(), p.n[2], w, p.ldc,
+                                            LoadCopy{}, s, rows, &pi);
   {
     ColView<T> v = y_view<T>(p, w);
     v.A = n * p.n[0];
-    launch_cols<T, -1>(p.n[1], v, p.tile[1], p.tw1.as<cplx<T>>(), s);
+    launch_cols_wide<T, -1, TI>(p.n[1], v, p.tile[1], pi.tw1.as<cplx<TI>>(),
+                                p.tw1.as<cplx<T>>(), s);
   }
   PxOut po{bt->pk.as<void>(), bt->q1.as<void>(), n, (int64_t)p.n[0] * p.n[1] * p.ldc, bt->npk,
            bt->nq1};

@@ -50,6 +50,22 @@ template <typename C>
 __device__ __forceinline__ C cmul(C a, C b) {
   return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
+// Complex value in another precision (the storage type of a pass whose
+// arithmetic runs in a wider type).
+template <typename To, typename C>
+__device__ __forceinline__ cplx<To> to_cplx(C a) {
+  return cplx<To>{To(a.x), To(a.y)};
+}
+template <typename C>
+struct ScalarOf;
+template <>
+struct ScalarOf<float2> {
+  using type = float;
+};
+template <>
+struct ScalarOf<double2> {
+  using type = double;
+};
 // a * (DIR * i): DIR = -1 multiplies by -i, DIR = +1 by +i.
 template <int DIR, typename C>
 __device__ __forceinline__ C mul_i(C a) {
@@ -294,7 +310,9 @@ struct LayRowPad {
 __host__ __device__ constexpr int row_pad_len(int M) { return M + M / 8 + 1; }
 
 // One Stockham stage on NT tiles in lock-step (one twiddle load serves every
-// tile; tile q starts ts elements after tile q - 1).  FIRST: the inputs are
+// tile; tile q starts ts elements after tile q - 1).  Arithmetic in T; the
+// LDS tiles hold CL values (T, or the narrower storage type of a pass whose
+// arithmetic is wider: each exchange then rounds once).  FIRST: the inputs are
 // already in v (no LDS read).  LAST: the outputs stay in v (no LDS write).  A
 // stage that reads and then writes LDS synchronises in between; a stage that
 // writes LDS ends with a barrier.  A FIRST stage writes without a barrier
@@ -305,9 +323,10 @@ __host__ __device__ constexpr int row_pad_len(int M) { return M + M / 8 + 1; }
 // entries at OFF, entry k * R + r = exp(-2 pi i r k / (NS R)), so a thread's
 // R - 1 twiddles of one stage are contiguous.
 template <typename T, int L, int R, int NS, int OFF, int DIR, int NT, bool FIRST, bool LAST,
-          class Lay>
-__device__ __forceinline__ void rstage(cplx<T> (&v)[NT][FftCfg<L>::E], cplx<T>* lds, int ts,
+          class Lay, int Q0 = 0, int NQ = NT, class CL = cplx<T>>
+__device__ __forceinline__ void rstage(cplx<T> (&v)[NT][FftCfg<L>::E], CL* lds, int ts,
                                        const cplx<T>* __restrict__ tws, int c, int t, Lay lay) {
+  using TL = typename ScalarOf<CL>::type;
   using C = cplx<T>;
   constexpr int E = FftCfg<L>::E;
   constexpr int TCOL = L / E;
@@ -315,11 +334,12 @@ __device__ __forceinline__ void rstage(cplx<T> (&v)[NT][FftCfg<L>::E], cplx<T>* 
   constexpr int LR = L / R;
   if (!FIRST) {
 #pragma unroll
-    for (int q = 0; q < NT; ++q)
+    for (int q = Q0; q < Q0 + NQ; ++q)
 #pragma unroll
       for (int b = 0; b < NB; ++b)
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[q][b * R + r] = lds[q * ts + lay(c, t + b * TCOL + r * LR)];
+        for (int r = 0; r < R; ++r)
+          v[q][b * R + r] = to_cplx<T>(lds[(q - Q0) * ts + lay(c, t + b * TCOL + r * LR)]);
     if (!LAST) __syncthreads();
   }
 #pragma unroll
@@ -333,42 +353,46 @@ __device__ __forceinline__ void rstage(cplx<T> (&v)[NT][FftCfg<L>::E], cplx<T>* 
         C w = wk[r];
         if (DIR > 0) w.y = -w.y;
 #pragma unroll
-        for (int q = 0; q < NT; ++q) v[q][b * R + r] = cmul(v[q][b * R + r], w);
+        for (int q = Q0; q < Q0 + NQ; ++q) v[q][b * R + r] = cmul(v[q][b * R + r], w);
       }
     }
 #pragma unroll
-    for (int q = 0; q < NT; ++q) Bfly<R, DIR, T>::run(&v[q][b * R]);
+    for (int q = Q0; q < Q0 + NQ; ++q) Bfly<R, DIR, T>::run(&v[q][b * R]);
     if (!LAST) {
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int q = 0; q < NT; ++q)
+      for (int q = Q0; q < Q0 + NQ; ++q)
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[q * ts + lay(c, base + r * NS)] = v[q][b * R + r];
+        for (int r = 0; r < R; ++r)
+          lds[(q - Q0) * ts + lay(c, base + r * NS)] = to_cplx<TL>(v[q][b * R + r]);
     }
   }
   if (!LAST) __syncthreads();
 }
 
-template <typename T, int L, int DIR, int NT, int NS, int OFF, bool FIRST, class Lay>
-__device__ __forceinline__ void rstages_(Radices<>, cplx<T> (&)[NT][FftCfg<L>::E], cplx<T>*, int,
+template <typename T, int L, int DIR, int NT, int NS, int OFF, bool FIRST, int Q0, int NQ,
+          class Lay, class CL>
+__device__ __forceinline__ void rstages_(Radices<>, cplx<T> (&)[NT][FftCfg<L>::E], CL*, int,
                                          const cplx<T>*, int, int, Lay) {}
 
-template <typename T, int L, int DIR, int NT, int NS, int OFF, bool FIRST, class Lay, int R,
-          int... Rest>
+template <typename T, int L, int DIR, int NT, int NS, int OFF, bool FIRST, int Q0, int NQ,
+          class Lay, class CL, int R, int... Rest>
 __device__ __forceinline__ void rstages_(Radices<R, Rest...>, cplx<T> (&v)[NT][FftCfg<L>::E],
-                                         cplx<T>* lds, int ts, const cplx<T>* __restrict__ tws,
+                                         CL* lds, int ts, const cplx<T>* __restrict__ tws,
                                          int c, int t, Lay lay) {
-  rstage<T, L, R, NS, OFF, DIR, NT, FIRST, sizeof...(Rest) == 0>(v, lds, ts, tws, c, t, lay);
-  rstages_<T, L, DIR, NT, NS * R, (NS > 1 ? OFF + NS * R : OFF), false>(Radices<Rest...>{}, v,
-                                                                        lds, ts, tws, c, t, lay);
+  rstage<T, L, R, NS, OFF, DIR, NT, FIRST, sizeof...(Rest) == 0, Lay, Q0, NQ, CL>(v, lds, ts,
+                                                                                   tws, c, t, lay);
+  rstages_<T, L, DIR, NT, NS * R, (NS > 1 ? OFF + NS * R : OFF), false, Q0, NQ, Lay, CL>(
+      Radices<Rest...>{}, v, lds, ts, tws, c, t, lay);
 }
 
 // All stages; tw is the axis's twiddle table (plain part then stage part).
-template <typename T, int L, int DIR, int NT, int NS, bool FIRST, class Lay, class Rad>
-__device__ __forceinline__ void rstages(Rad, cplx<T> (&v)[NT][FftCfg<L>::E], cplx<T>* lds,
-                                        int ts, const cplx<T>* __restrict__ tw, int c, int t,
-                                        Lay lay) {
-  rstages_<T, L, DIR, NT, NS, 0, FIRST>(Rad{}, v, lds, ts, tw + L, c, t, lay);
+// Tiles Q0 .. Q0 + NQ - 1 of v run in lock-step (LDS tile q - Q0).
+template <typename T, int L, int DIR, int NT, int NS, bool FIRST, class Lay, class Rad,
+          int Q0 = 0, int NQ = NT, class CL>
+__device__ __forceinline__ void rstages(Rad, cplx<T> (&v)[NT][FftCfg<L>::E], CL* lds, int ts,
+                                        const cplx<T>* __restrict__ tw, int c, int t, Lay lay) {
+  rstages_<T, L, DIR, NT, NS, 0, FIRST, Q0, NQ, Lay, CL>(Rad{}, v, lds, ts, tw + L, c, t, lay);
 }
 
 // Host: length of the per-stage table of an L-point register-chained plan,
@@ -406,10 +430,10 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
 // TC columns per workgroup, one launch of A * G * ceil(B / TC) workgroups.
 // Output o < nout holds element (o - off) mod L (offm = off mod L; nout may
 // exceed L).  In place when in == out: every workgroup reads its whole tile
-// before writing it.
-template <typename T, int L, int DIR, int TC, typename Tin = T>
+// before writing it.  Arithmetic in T; loads Tin; LDS and stores in To.
+template <typename T, int L, int DIR, int TC, typename Tin = T, typename To = T>
 __global__ void __launch_bounds__(TC*(L / FftCfg<L>::E))
-    k_fft_cols_rc(const cplx<Tin>* in, int64_t inA, int64_t inL, int64_t inG, cplx<T>* out,
+    k_fft_cols_rc(const cplx<Tin>* in, int64_t inA, int64_t inL, int64_t inG, cplx<To>* out,
                   int64_t outA, int64_t outL, int64_t outG, int B, int tilesB, int G, int nout,
                   int offm, const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
@@ -418,7 +442,7 @@ __global__ void __launch_bounds__(TC*(L / FftCfg<L>::E))
   using Rad = typename FftCfg<L>::radices;
   static_assert(RadixInfo<Rad>::first == E, "first radix must equal E");
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* lds = reinterpret_cast<C*>(smem_raw);
+  cplx<To>* lds = reinterpret_cast<cplx<To>*>(smem_raw);
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int ag = tile / tilesB;
   const int b0 = (tile - ag * tilesB) * TC;
@@ -426,7 +450,7 @@ __global__ void __launch_bounds__(TC*(L / FftCfg<L>::E))
   const int c = threadIdx.x % TC, t = threadIdx.x / TC;
   const bool colok = b0 + c < B;
   const cplx<Tin>* src = in + (int64_t)a * inA + (int64_t)g * inG + b0 + c;
-  C* dst = out + (int64_t)a * outA + (int64_t)g * outG + b0 + c;
+  cplx<To>* dst = out + (int64_t)a * outA + (int64_t)g * outG + b0 + c;
   C v[1][E];
 #pragma unroll
   for (int s = 0; s < E; ++s) {
@@ -443,7 +467,7 @@ __global__ void __launch_bounds__(TC*(L / FftCfg<L>::E))
   for (int s = 0; s < E; ++s) {
     int o = t + out_m<L>(s) * TCOL + offm;
     if (o >= L) o -= L;
-    for (; o < nout; o += L) dst[(int64_t)o * outL] = v[0][s];
+    for (; o < nout; o += L) dst[(int64_t)o * outL] = to_cplx<To>(v[0][s]);
   }
 }
 
@@ -462,9 +486,10 @@ struct PairOf<double> {
 // Register-chained R2C rows: a real row of 2M samples packed as M complex
 // values, a length-M FFT, then the Hermitian post-process (see
 // k_fft_rows_r2c).  Block = tr rows x M/E threads, LDS tr row_pad_len(M).
-template <typename T, typename Tin, int M, class Loader>
+// Arithmetic in T; LDS and outputs in To.
+template <typename T, typename Tin, int M, class Loader, typename To = T>
 __global__ void __launch_bounds__(1024)
-    k_fft_rows_r2c_rc(const Tin* in, int64_t ldi, cplx<T>* out, int64_t ldo, int64_t nrows,
+    k_fft_rows_r2c_rc(const Tin* in, int64_t ldi, cplx<To>* out, int64_t ldo, int64_t nrows,
                       int tr, const cplx<T>* __restrict__ twM, const cplx<T>* __restrict__ twN,
                       Loader ld) {
   using C = cplx<T>;
@@ -474,7 +499,7 @@ __global__ void __launch_bounds__(1024)
   using Rad = typename FftCfg<M>::radices;
   static_assert(RadixInfo<Rad>::first == E, "first radix must equal E");
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* lds = reinterpret_cast<C*>(smem_raw);
+  cplx<To>* lds = reinterpret_cast<cplx<To>*>(smem_raw);
   const LayRowPad lay{RS};
   const int c = threadIdx.x / TCOL, t = threadIdx.x % TCOL;
   const int64_t row = (int64_t)blockIdx.x * tr + c;
@@ -498,20 +523,20 @@ __global__ void __launch_bounds__(1024)
   rstages<T, M, -1, 1, 1, true>(Rad{}, v, lds, 0, twM, c, t, lay);
   __syncthreads();  // the last stage's LDS reads precede the stores below
 #pragma unroll
-  for (int s = 0; s < E; ++s) lds[lay(c, t + out_m<M>(s) * TCOL)] = v[0][s];
+  for (int s = 0; s < E; ++s) lds[lay(c, t + out_m<M>(s) * TCOL)] = to_cplx<To>(v[0][s]);
   __syncthreads();
   // Hermitian half: X[k] = (Z[k] + conj Z[M-k])/2 - i W^k (Z[k] - conj Z[M-k])/2
   if (!rowok) return;
-  C* dst = out + row * ldo;
+  cplx<To>* dst = out + row * ldo;
   for (int k = t; k <= M; k += TCOL) {
-    const C zk = lds[lay(c, k == M ? 0 : k)];
-    const C zm = lds[lay(c, k == 0 ? 0 : M - k)];
+    const C zk = to_cplx<T>(lds[lay(c, k == M ? 0 : k)]);
+    const C zm = to_cplx<T>(lds[lay(c, k == 0 ? 0 : M - k)]);
     const T ax = zk.x + zm.x, ay = zk.y - zm.y;
     const T bx = zk.x - zm.x, by = zk.y + zm.y;
     const C w = twN[k];
     const T px = w.x * by + w.y * bx;
     const T py = -(w.x * bx - w.y * by);
-    dst[k] = C{T(0.5) * (ax + px), T(0.5) * (ay + py)};
+    dst[k] = to_cplx<To>(C{T(0.5) * (ax + px), T(0.5) * (ay + py)});
   }
 }
 

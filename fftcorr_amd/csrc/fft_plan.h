@@ -5,6 +5,8 @@
 // callers can fuse their own loads into the z pass.
 #pragma once
 
+#include <type_traits>
+
 #include "fc_internal.h"
 #include "fft_kernels.h"
 
@@ -65,8 +67,10 @@ struct ColView {
   int nout = 0, off = 0;  // nout == 0: all L outputs
 };
 
-template <typename T, int DIR, typename Tin, int TC>
-void launch_cols_rc(int L, const ColView<T, Tin>& v, const cplx<T>* tw, hipStream_t s) {
+// Register-chained column pass: arithmetic in A (twiddles tw in A), loads
+// Tin, LDS and stores in T.
+template <typename T, int DIR, typename Tin, int TC, typename A = T>
+void launch_cols_rc(int L, const ColView<T, Tin>& v, const cplx<A>* tw, hipStream_t s) {
   const size_t cs = sizeof(cplx<T>);
   const int nout = v.nout ? v.nout : L;
   const int offm = ((v.off % L) + L) % L;
@@ -75,7 +79,7 @@ void launch_cols_rc(int L, const ColView<T, Tin>& v, const cplx<T>* tw, hipStrea
   if (nb > INT32_MAX) fail(FC_ERR_INVALID, "transform too large for one launch");
 #define FC_COLS_RC_CASE(Lv)                                                                  \
   case Lv:                                                                                   \
-    hipLaunchKernelGGL((k_fft_cols_rc<T, Lv, DIR, TC, Tin>), dim3((unsigned)nb),             \
+    hipLaunchKernelGGL((k_fft_cols_rc<A, Lv, DIR, TC, Tin, T>), dim3((unsigned)nb),          \
                        dim3(TC * (Lv / FftCfg<Lv>::E)), (size_t)Lv * TC * cs, s, v.in, v.inA, \
                        v.inL, v.inG, v.out, v.outA, v.outL, v.outG, v.B, tilesB, v.G, nout,  \
                        offm, tw);                                                            \
@@ -143,6 +147,23 @@ void launch_cols(int L, const ColView<T, Tin>& v, int tc, const cplx<T>* tw, hip
   FC_LAUNCH_CHECK();
 }
 
+// Column pass stored in T with arithmetic in the wider A where the
+// register-chained kernel takes L (tw_a: twiddles in A; at most 8 columns
+// per workgroup, as the wider registers allow fewer waves), else in T (tw_t).
+template <typename T, int DIR, typename A>
+void launch_cols_wide(int L, const ColView<T>& v, int tc, const cplx<A>* tw_a,
+                      const cplx<T>* tw_t, hipStream_t s) {
+  if constexpr (!std::is_same<T, A>::value) {
+    int t = tc < 8 ? tc : 8;
+    while (t > 4 && ((size_t)L * t * sizeof(cplx<T>) > kLdsBudget || t * (L / 8) > 1024)) t /= 2;
+    if (rc_len_ok(L) && (size_t)L * t * sizeof(cplx<T>) <= kLdsBudget && t * (L / 8) <= 1024) {
+      if (t == 8) return launch_cols_rc<T, DIR, T, 8, A>(L, v, tw_a, s);
+      return launch_cols_rc<T, DIR, T, 4, A>(L, v, tw_a, s);
+    }
+  }
+  launch_cols<T, DIR>(L, v, tc, tw_t, s);
+}
+
 // In-place y pass over the work grid [n0][n1][ldc].
 template <typename T>
 ColView<T> y_view(const FftPlan& p, cplx<T>* d) {
@@ -158,9 +179,12 @@ ColView<T> x_view(const FftPlan& p, cplx<T>* d) {
                     1, p.n[1], p.n2h};
 }
 
-template <typename T, typename Tin, class Loader>
+// z pass (R2C rows) stored in T.  With A wider than T, the register-chained
+// kernel computes in A with the twiddles of the A-precision plan *pa (same
+// shape), exchanging through LDS in T; lengths it does not take compute in T.
+template <typename T, typename Tin, class Loader, typename A = T>
 void launch_rows_r2c(const FftPlan& p, const Tin* in, int64_t ldi, cplx<T>* out, int64_t ldo,
-                     Loader ld, hipStream_t s, int64_t nrows = -1) {
+                     Loader ld, hipStream_t s, int64_t nrows = -1, const FftPlan* pa = nullptr) {
   const size_t cs = sizeof(cplx<T>);
   if (nrows < 0) nrows = p.nrows();
   const int M = p.n[2] / 2;
@@ -168,15 +192,18 @@ void launch_rows_r2c(const FftPlan& p, const Tin* in, int64_t ldi, cplx<T>* out,
   const cplx<T>* twM = p.tw2M.as<cplx<T>>();
   const cplx<T>* twN = p.tw2N.as<cplx<T>>();
   if (p.fast[2] && rc_len_ok(M)) {
+    const FftPlan& pt = std::is_same<T, A>::value ? p : *pa;
+    const cplx<A>* twMa = pt.tw2M.as<cplx<A>>();
+    const cplx<A>* twNa = pt.tw2N.as<cplx<A>>();
     int trc = tr;
     while (trc > 1 && (size_t)trc * row_pad_len(M) * cs > kLdsBudget) trc /= 2;
 #define FC_R2C_RC_CASE(Mv)                                                                   \
   case Mv: {                                                                                 \
     const int thr = trc * (Mv / FftCfg<Mv>::E);                                              \
     dim3 g((unsigned)((nrows + trc - 1) / trc));                                             \
-    hipLaunchKernelGGL((k_fft_rows_r2c_rc<T, Tin, Mv, Loader>), g, dim3(thr),                \
+    hipLaunchKernelGGL((k_fft_rows_r2c_rc<A, Tin, Mv, Loader, T>), g, dim3(thr),             \
                        (size_t)trc * row_pad_len(Mv) * cs, s, in, ldi, out, ldo, nrows, trc,  \
-                       twM, twN, ld);                                                        \
+                       twMa, twNa, ld);                                                      \
     break;                                                                                   \
   }
     switch (M) {

@@ -66,9 +66,11 @@ template <typename T>
 constexpr int px_waves() {
   return sizeof(T) == 4 ? 4 : 2;
 }
+// The forward tiles hold the storage type T; the inverse tile holds TI.
 template <typename T, typename TI, int L, int MODE>
 constexpr size_t px_lds_bytes() {
-  constexpr size_t fwd = (MODE == kModeInterlaced ? 2 : 1) * (size_t)L * px_tc<T>() * sizeof(cplx<T>);
+  constexpr int tiles = MODE == kModeInterlaced ? 2 : 1;
+  constexpr size_t fwd = tiles * (size_t)L * px_tc<T>() * sizeof(cplx<T>);
   constexpr size_t inv = (size_t)L * px_tc<T>() * sizeof(cplx<TI>);
   return fwd > inv ? fwd : inv;
 }
@@ -80,12 +82,13 @@ constexpr size_t px_lds_bytes() {
 // member b reads A, B at b * a_bstride and writes pk, q at b * pk_bstride,
 // b * q_bstride (elements).  In cross mode B is the one shared grid-2
 // spectrum (a_bstride does not apply to it).
-//   A: grid a after its z and y passes (forward precision T).
+//   A: grid a after its z and y passes (stored as T).
 //   B: grid b likewise (interlaced) or the fully transformed grid 2 (cross).
-//   pk: compact k-subgrid values (T); q[sx][y][c] (row stride ldq, inverse
-//   precision TI): inverse x FFT at element (sx - off) mod L for every
-//   sx < rs (offm = off mod L; rs may exceed L, then elements repeat).
-// Forward in T; combine, inverse and q in TI (FP64 for the mixed form).
+//   pk: compact k-subgrid values (T); q[sx][y][c] (row stride ldq, stored
+//   as TI): inverse x FFT at element (sx - off) mod L for every sx < rs
+//   (offm = off mod L; rs may exceed L, then elements repeat).
+// Loads, forward FFT (twiddles tw) and pk in T; combine, inverse and q in
+// TI (FP64 for the mixed form), twiddles twi.
 template <typename T, typename TI, int L, int MODE>
 __global__ void __launch_bounds__((px_threads<T, L>()))
 __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
@@ -103,6 +106,7 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
   constexpr int TCOL = L / E;
   constexpr int NT = MODE == kModeInterlaced ? 2 : 1;
   using Rad = typename FftCfg<L>::radices;
+  using Lay = LaySwz<TC>;
   static_assert(RadixInfo<Rad>::first == E, "first radix must equal E");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
@@ -143,7 +147,7 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
       g[s] = colok ? *reinterpret_cast<const C*>(Bb + o) : C{T(0), T(0)};
     }
   }
-  rstages<T, L, -1, NT, 1, true>(Rad{}, v, lds, L * TC, tw, c, t, LaySwz<TC>{});
+  rstages<T, L, -1, NT, 1, true>(Rad{}, v, lds, L * TC, tw, c, t, Lay{});
 
   // Combine: slot s holds element e = t + out_m(s) L/E of both spectra.
   const int oz = ks.oz();
@@ -195,7 +199,7 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
     }
   }
   __syncthreads();  // the forward pass's last LDS reads precede the inverse's stores
-  rstages<TI, L, +1, 1, 1, true>(Rad{}, u, ldsi, 0, twi, c, t, LaySwz<TC>{});
+  rstages<TI, L, +1, 1, 1, true>(Rad{}, u, ldsi, 0, twi, c, t, Lay{});
   if (colok) {
 #pragma unroll
     for (int s = 0; s < E; ++s) {

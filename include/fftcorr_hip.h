@@ -15,7 +15,9 @@
  *                        3 CIC, 4 CIC aliased (new)   cc/correlate/window_correct.h:8-12
  * Precision: FC_FP64 computes every stage in FP64 (the reference's Float);
  * FC_FP32 keeps grids, FFTs and products in FP32 with FP64 sums, Y_lm
- * normalisation and histogram accumulation; FC_MIXED is FC_FP32 with the
+ * normalisation and histogram accumulation; FC_MIXED stores grids and
+ * spectra in FP32 like FC_FP32 but computes the z and y forward FFT passes in
+ * FP64 (values rounded to FP32 only where they go to LDS or HBM) and the
  * (pruned) inverse FFT of the power/cross spectrum in FP64.
  */
 #ifndef FFTCORR_HIP_H
