@@ -348,12 +348,27 @@ __device__ __forceinline__ void rstage(cplx<T> (&v)[NT][FftCfg<L>::E], CL* lds, 
     const int k = j % NS;
     if (NS > 1) {
       const C* wk = tws + OFF + k * R;
+      if constexpr (sizeof(T) == 8) {
+        // FP64: entry r is w^r, w = entry 1; the powers are formed in
+        // registers (error ~ r ulp of FP64), which keeps one twiddle live
+        // instead of R - 1.
+        C w1 = wk[1];
+        if (DIR > 0) w1.y = -w1.y;
+        C w = w1;
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        C w = wk[r];
-        if (DIR > 0) w.y = -w.y;
+        for (int r = 1; r < R; ++r) {
 #pragma unroll
-        for (int q = Q0; q < Q0 + NQ; ++q) v[q][b * R + r] = cmul(v[q][b * R + r], w);
+          for (int q = Q0; q < Q0 + NQ; ++q) v[q][b * R + r] = cmul(v[q][b * R + r], w);
+          if (r + 1 < R) w = cmul(w, w1);
+        }
+      } else {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          C w = wk[r];
+          if (DIR > 0) w.y = -w.y;
+#pragma unroll
+          for (int q = Q0; q < Q0 + NQ; ++q) v[q][b * R + r] = cmul(v[q][b * R + r], w);
+        }
       }
     }
 #pragma unroll
@@ -431,8 +446,16 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
 // Output o < nout holds element (o - off) mod L (offm = off mod L; nout may
 // exceed L).  In place when in == out: every workgroup reads its whole tile
 // before writing it.  Arithmetic in T; loads Tin; LDS and stores in To.
+// Waves per SIMD a register-chained column kernel is held to when its
+// arithmetic is wider than its LDS tiles (the occupancy of the narrow form).
+template <typename T, typename To>
+constexpr int rc_wide_waves() {
+  return sizeof(T) > sizeof(To) ? 8 : 1;
+}
+
 template <typename T, int L, int DIR, int TC, typename Tin = T, typename To = T>
 __global__ void __launch_bounds__(TC*(L / FftCfg<L>::E))
+__attribute__((amdgpu_waves_per_eu(rc_wide_waves<T, To>())))
     k_fft_cols_rc(const cplx<Tin>* in, int64_t inA, int64_t inL, int64_t inG, cplx<To>* out,
                   int64_t outA, int64_t outL, int64_t outG, int B, int tilesB, int G, int nout,
                   int offm, const cplx<T>* __restrict__ tw) {
@@ -449,13 +472,21 @@ __global__ void __launch_bounds__(TC*(L / FftCfg<L>::E))
   const int a = ag / G, g = ag - a * G;
   const int c = threadIdx.x % TC, t = threadIdx.x / TC;
   const bool colok = b0 + c < B;
-  const cplx<Tin>* src = in + (int64_t)a * inA + (int64_t)g * inG + b0 + c;
-  cplx<To>* dst = out + (int64_t)a * outA + (int64_t)g * outG + b0 + c;
+  // Uniform bases and 32-bit per-thread byte offsets (the launcher checks
+  // that a tile's offsets stay below 4 GB).
+  const char* srcb =
+      reinterpret_cast<const char*>(in + (int64_t)a * inA + (int64_t)g * inG + b0);
+  char* dstb = reinterpret_cast<char*>(out + (int64_t)a * outA + (int64_t)g * outG + b0);
+  const uint32_t inLb = (uint32_t)inL * (uint32_t)sizeof(cplx<Tin>);
+  const uint32_t outLb = (uint32_t)outL * (uint32_t)sizeof(cplx<To>);
+  const uint32_t cin = (uint32_t)c * (uint32_t)sizeof(cplx<Tin>);
+  const uint32_t cout = (uint32_t)c * (uint32_t)sizeof(cplx<To>);
   C v[1][E];
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     if (colok) {
-      const cplx<Tin> x = src[(int64_t)(t + s * TCOL) * inL];
+      const cplx<Tin> x =
+          *reinterpret_cast<const cplx<Tin>*>(srcb + (cin + (uint32_t)(t + s * TCOL) * inLb));
       v[0][s] = C{T(x.x), T(x.y)};
     } else {
       v[0][s] = C{T(0), T(0)};
@@ -467,8 +498,20 @@ __global__ void __launch_bounds__(TC*(L / FftCfg<L>::E))
   for (int s = 0; s < E; ++s) {
     int o = t + out_m<L>(s) * TCOL + offm;
     if (o >= L) o -= L;
-    for (; o < nout; o += L) dst[(int64_t)o * outL] = to_cplx<To>(v[0][s]);
+    for (; o < nout; o += L)
+      *reinterpret_cast<cplx<To>*>(dstb + (cout + (uint32_t)o * outLb)) = to_cplx<To>(v[0][s]);
   }
+}
+
+// Whether a column view's per-tile byte offsets fit the 32-bit offsets of
+// k_fft_cols_rc.
+template <typename To, typename Tin>
+bool rc_offsets_fit(int L, int tc, int64_t inL, int64_t outL, int nout) {
+  const uint64_t lim = 0xffffffffull;
+  const uint64_t in_span = ((uint64_t)tc + (uint64_t)(L - 1) * (uint64_t)inL) * sizeof(cplx<Tin>);
+  const uint64_t out_span =
+      ((uint64_t)tc + (uint64_t)(nout - 1) * (uint64_t)outL) * sizeof(cplx<To>);
+  return in_span < lim && out_span < lim;
 }
 
 // Loads the real sample pair (2e, 2e + 1) of a row as one vector access.
@@ -528,15 +571,20 @@ __global__ void __launch_bounds__(1024)
   // Hermitian half: X[k] = (Z[k] + conj Z[M-k])/2 - i W^k (Z[k] - conj Z[M-k])/2
   if (!rowok) return;
   cplx<To>* dst = out + row * ldo;
+  // FP64: W^k for k = t, t + TCOL, ... by repeated products with W^TCOL
+  // (error ~ M / TCOL ulp of FP64) rather than one table load per k.
+  C w = twN[t];
+  const C wstep = twN[TCOL];
   for (int k = t; k <= M; k += TCOL) {
     const C zk = to_cplx<T>(lds[lay(c, k == M ? 0 : k)]);
     const C zm = to_cplx<T>(lds[lay(c, k == 0 ? 0 : M - k)]);
     const T ax = zk.x + zm.x, ay = zk.y - zm.y;
     const T bx = zk.x - zm.x, by = zk.y + zm.y;
-    const C w = twN[k];
+    if constexpr (sizeof(T) != 8) w = twN[k];
     const T px = w.x * by + w.y * bx;
     const T py = -(w.x * bx - w.y * by);
     dst[k] = to_cplx<To>(C{T(0.5) * (ax + px), T(0.5) * (ay + py)});
+    if constexpr (sizeof(T) == 8) w = cmul(w, wstep);
   }
 }
 

@@ -103,7 +103,8 @@ template <typename T, int DIR, typename Tin = T>
 void launch_cols(int L, const ColView<T, Tin>& v, int tc, const cplx<T>* tw, hipStream_t s) {
   const size_t cs = sizeof(cplx<T>);
   const int nout = v.nout ? v.nout : L;
-  if (rc_len_ok(L) && (size_t)L * tc * cs <= kLdsBudget && tc * (L / 8) <= 1024) {
+  if (rc_len_ok(L) && (size_t)L * tc * cs <= kLdsBudget && tc * (L / 8) <= 1024 &&
+      rc_offsets_fit<T, Tin>(L, tc, v.inL, v.outL, nout)) {
     if (tc == 16) return launch_cols_rc<T, DIR, Tin, 16>(L, v, tw, s);
     if (tc == 8) return launch_cols_rc<T, DIR, Tin, 8>(L, v, tw, s);
     if (tc == 4) return launch_cols_rc<T, DIR, Tin, 4>(L, v, tw, s);
@@ -148,15 +149,16 @@ void launch_cols(int L, const ColView<T, Tin>& v, int tc, const cplx<T>* tw, hip
 }
 
 // Column pass stored in T with arithmetic in the wider A where the
-// register-chained kernel takes L (tw_a: twiddles in A; at most 8 columns
-// per workgroup, as the wider registers allow fewer waves), else in T (tw_t).
+// register-chained kernel takes L (tw_a: twiddles in A), else in T (tw_t).
 template <typename T, int DIR, typename A>
 void launch_cols_wide(int L, const ColView<T>& v, int tc, const cplx<A>* tw_a,
                       const cplx<T>* tw_t, hipStream_t s) {
   if constexpr (!std::is_same<T, A>::value) {
-    int t = tc < 8 ? tc : 8;
+    int t = tc;
     while (t > 4 && ((size_t)L * t * sizeof(cplx<T>) > kLdsBudget || t * (L / 8) > 1024)) t /= 2;
-    if (rc_len_ok(L) && (size_t)L * t * sizeof(cplx<T>) <= kLdsBudget && t * (L / 8) <= 1024) {
+    if (rc_len_ok(L) && (size_t)L * t * sizeof(cplx<T>) <= kLdsBudget && t * (L / 8) <= 1024 &&
+        rc_offsets_fit<T, T>(L, t, v.inL, v.outL, v.nout ? v.nout : L)) {
+      if (t >= 16) return launch_cols_rc<T, DIR, T, 16, A>(L, v, tw_a, s);
       if (t == 8) return launch_cols_rc<T, DIR, T, 8, A>(L, v, tw_a, s);
       return launch_cols_rc<T, DIR, T, 4, A>(L, v, tw_a, s);
     }
