@@ -8,7 +8,8 @@ namespace fc {
 
 void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
                         const std::vector<double>& ay, const std::vector<double>& az,
-                        double binsize, int nbins, int64_t track_cell) {
+                        double binsize, int nbins, int64_t track_cell,
+                        const std::function<uint64_t(uint32_t)>* order) {
   const int s0 = (int)ax.size(), s1 = (int)ay.size(), s2 = (int)az.size();
   if ((uint64_t)s0 * s1 * s2 > 0xffffffffULL) fail(FC_ERR_INVALID, "subgrid too large");
   // Bin of cell (i, j, k): floor((|r| - 0) / binsize) with
@@ -74,6 +75,22 @@ void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
                 cells[o[b]++] = (uint32_t)(((int64_t)i * s1 + j) * s2 + k);
               }
             }
+      });
+    for (auto& x : th) x.join();
+  }
+  if (order) {
+    // Each bin's segment sorted by (order key, cell); bins split over threads.
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+      th.emplace_back([&, t] {
+        std::vector<std::pair<uint64_t, uint32_t>> tmp;
+        for (int b = (int)t; b < nbins; b += (int)nt) {
+          const int64_t lo = start[b], hi = start[b + 1];
+          tmp.resize(hi - lo);
+          for (int64_t q = lo; q < hi; ++q) tmp[q - lo] = {(*order)(cells[q]), cells[q]};
+          std::sort(tmp.begin(), tmp.end());
+          for (int64_t q = lo; q < hi; ++q) cells[q] = tmp[q - lo].second;
+        }
       });
     for (auto& x : th) x.join();
   }

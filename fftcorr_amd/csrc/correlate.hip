@@ -512,7 +512,18 @@ void setup_geometry(fc_corr* c) {
     w[i].resize(c->kshape[i]);
     for (int k = 0; k < c->kshape[i]; ++k) w[i][k] = window_sq(c->window, (*axes[i])[k], c->cell);
   }
-  build_binned_cells(c->kbin, c->h_kx, c->h_ky, c->h_kz, c->dk, c->nkbins);
+  // k cells ordered within a bin by the compact k value they read (see
+  // KEntry): the two Hermitian partners of a value become neighbours.
+  const KSlots ks{c->kshape[0], c->kshape[1], c->kshape[2]};
+  const std::function<uint64_t(uint32_t)> kkey = [ks](uint32_t cell) -> uint64_t {
+    const int k = (int)(cell % (uint32_t)ks.ks2);
+    const uint32_t t = cell / (uint32_t)ks.ks2;
+    const int j = (int)(t % (uint32_t)ks.ks1), i = (int)(t / (uint32_t)ks.ks1);
+    const int oz = ks.oz();
+    return (uint64_t)(k >= oz ? ks.at(i, j, k - oz)
+                              : ks.at(ks.ks0 - 1 - i, ks.ks1 - 1 - j, oz - k));
+  };
+  build_binned_cells(c->kbin, c->h_kx, c->h_ky, c->h_kz, c->dk, c->nkbins, -1, &kkey);
   const int64_t o = c->rmax_cells;
   build_binned_cells(c->rbin, c->h_rx, c->h_rx, c->h_rx, c->dr, c->nrbins,
                      (o * c->rshape + o) * c->rshape + o);

@@ -9,6 +9,8 @@
 // order therefore depends only on the problem shape, never on scheduling.
 #pragma once
 
+#include <functional>
+
 #include "fc_objects.h"
 
 namespace fc {
@@ -117,9 +119,13 @@ struct BinnedCells {
 };
 
 // track_cell >= 0: record the entry position of that cell in tracked_pos.
+// order (optional): within each bin, cells are ordered by (order(cell), cell)
+// instead of raster order -- e.g. by the address the histogram kernel reads
+// for the cell, so that neighbouring entries read neighbouring values.
 void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
                         const std::vector<double>& ay, const std::vector<double>& az,
-                        double binsize, int nbins, int64_t track_cell = -1);
+                        double binsize, int nbins, int64_t track_cell = -1,
+                        const std::function<uint64_t(uint32_t)>* order = nullptr);
 
 // Unsigned 32-bit division by a runtime constant d: q = (umulhi(x, m) + x) >> s
 // with the round-up magic m = floor(2^32 (2^s - d) / d) + 1, s = ceil(log2 d)

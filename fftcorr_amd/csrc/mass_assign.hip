@@ -106,17 +106,27 @@ __global__ void __launch_bounds__(kPrepThreads)
   }
 }
 
-__global__ void k_ma_finish_stats(const double* __restrict__ part_w,
-                                  const unsigned long long* __restrict__ part_n, int nb,
-                                  double* __restrict__ out_w, unsigned long long* out_n) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double a = 0.0, b = 0.0;
-    unsigned long long c = 0;
-    for (int i = 0; i < nb; ++i) {
-      a += part_w[2 * i];
-      b += part_w[2 * i + 1];
-      c += part_n[i];
-    }
+// One block of kWave threads: lane l sums partials l, l + 64, ... in order,
+// then the fixed butterfly of block_sum; the result does not depend on
+// scheduling.
+__global__ void __launch_bounds__(kWave)
+    k_ma_finish_stats(const double* __restrict__ part_w,
+                      const unsigned long long* __restrict__ part_n, int nb,
+                      double* __restrict__ out_w, unsigned long long* out_n) {
+#pragma clang fp contract(off)
+  __shared__ double sd[1];
+  __shared__ unsigned long long su[1];
+  double a = 0.0, b = 0.0;
+  unsigned long long c = 0;
+  for (int i = threadIdx.x; i < nb; i += kWave) {
+    a += part_w[2 * i];
+    b += part_w[2 * i + 1];
+    c += part_n[i];
+  }
+  a = block_sum(a, sd);
+  b = block_sum(b, sd);
+  c = block_sum_u64(c, su);
+  if (threadIdx.x == 0) {
     out_w[0] = a;
     out_w[1] = b;
     out_n[0] = c;
@@ -142,6 +152,15 @@ __global__ void k_cell_starts(const uint64_t* __restrict__ keys, const unsigned 
     const int64_t cur = i == nvalid ? (int64_t)ncells : (int64_t)keys[i];
     for (int64_t c = prev + 1; c <= cur; ++c) start[c] = (int32_t)i;
   }
+}
+
+// v mod n for the halo of a tile: one compare in the common case, the full
+// modulo only on grids smaller than the halo.
+__device__ __forceinline__ int wrap_index(int v, int n) {
+  if ((unsigned)v < (unsigned)n) return v;
+  if (v < 0 && v + n >= 0) return v + n;
+  if (v >= n && v - n < n) return v - n;
+  return ((v % n) + n) % n;
 }
 
 // 1D stencil weight for offset o (target = source + o) and in-cell offset r.
@@ -174,9 +193,9 @@ __global__ void __launch_bounds__(TX* TY* TZ)
   const int x0 = bx * TX, y0 = by * TY, z0 = bz * TZ;
   for (int h = threadIdx.x; h < HX * HY * HZ; h += blockDim.x) {
     const int hz = h % HZ, hy = (h / HZ) % HY, hx = h / (HZ * HY);
-    const int gx = ((x0 + hx - 1) % g.n0 + g.n0) % g.n0;
-    const int gy = ((y0 + hy - 1) % g.n1 + g.n1) % g.n1;
-    const int gz = ((z0 + hz - 1) % g.n2 + g.n2) % g.n2;
+    const int gx = wrap_index(x0 + hx - 1, g.n0);
+    const int gy = wrap_index(y0 + hy - 1, g.n1);
+    const int gz = wrap_index(z0 + hz - 1, g.n2);
     const int64_t c = ((int64_t)gx * g.n1 + gy) * g.n2 + gz;
     cs[h] = start[c];
     ce[h] = start[c + 1];
@@ -238,9 +257,9 @@ __global__ void __launch_bounds__(TX* TY* TZ)
   for (int i = threadIdx.x; i < WC * WS; i += blockDim.x) tab[i] = table[i];
   for (int h = threadIdx.x; h < DX * DY * DZ; h += blockDim.x) {
     const int hz = h % DZ, hy = (h / DZ) % DY, hx = h / (DZ * DY);
-    const int gx = ((x0 + hx - (WC - 2)) % g.n0 + g.n0) % g.n0;
-    const int gy = ((y0 + hy - (WC - 2)) % g.n1 + g.n1) % g.n1;
-    const int gz = ((z0 + hz - (WC - 2)) % g.n2 + g.n2) % g.n2;
+    const int gx = wrap_index(x0 + hx - (WC - 2), g.n0);
+    const int gy = wrap_index(y0 + hy - (WC - 2), g.n1);
+    const int gz = wrap_index(z0 + hz - (WC - 2), g.n2);
     const int64_t c = ((int64_t)gx * g.n1 + gy) * g.n2 + gz;
     cs[h] = start[c];
     ce[h] = start[c + 1];
