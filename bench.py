@@ -26,6 +26,12 @@ FP64 fused result per multipole (floor 1e-3 max|l=0|), bit-exact counts,
 p50/p99 latency, the dominant kernel's HBM roofline, and the FP64 CPU oracle
 (scipy pocketfft, all host threads) timed on one correlation.
 
+Timing: `value` is the throughput of the timed steps queued back to back on
+the correlator's stream (synchronised before and after); p50/p99 come from
+a second pass that times each correlation on its own (HIP events around
+it, the host waiting for each), and the per-stage times from a third pass
+with stage events.
+
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
 """
 
@@ -106,20 +112,31 @@ def run_form(lib, corr_mod, ctx, pos, cfg, precision, steps, warmup, profile,
     ga, gb = grids[0].handle, grids[1].handle
     for _ in range(warmup):
         lib.check(L.fc_corr_autocorrelate_interlaced(h, ga, gb))
-    ctx.synchronize()
-    if profile:
-        c.set_profiling(True)
-    lat = []
-    t = ctypes.c_double()
+    # Throughput: the timed steps queued back to back on the correlator's
+    # stream, synchronised before and after.
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         lib.check(L.fc_corr_autocorrelate_interlaced(h, ga, gb))
-        lib.check(L.fc_corr_last_time_ms(h, ctypes.byref(t)))
-        lat.append(t.value)
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    stages = c.stage_times() if profile else None
+    # Latency: each correlation on its own (device time between the events
+    # that bracket it), the host waiting for each before the next.
+    lat = []
+    t = ctypes.c_double()
+    for _ in range(steps):
+        lib.check(L.fc_corr_autocorrelate_interlaced(h, ga, gb))
+        lib.check(L.fc_corr_last_time_ms(h, ctypes.byref(t)))
+        lat.append(t.value)
+    # Per-stage device times (HIP events between the stages).
+    stages = None
+    if profile:
+        c.set_profiling(True)
+        for _ in range(steps):
+            lib.check(L.fc_corr_autocorrelate_interlaced(h, ga, gb))
+        ctx.synchronize()
+        stages = c.stage_times()
+        c.set_profiling(False)
     c._fetch()
     res = {k: np.array(v, copy=True) if isinstance(v, np.ndarray) else v
            for k, v in c.results().items()}

@@ -403,6 +403,20 @@ void fill(V* p, int64_t n, V v, hipStream_t s) {
   FC_LAUNCH_CHECK();
 }
 
+// Zeroes up to 5 arrays of 32-bit words in one launch (the per-correlation
+// histogram outputs), block y = array.
+struct ZeroSegs {
+  uint32_t* p[5];
+  int64_t n[5];
+};
+__global__ void k_zero_segs(ZeroSegs z) {
+  uint32_t* p = z.p[blockIdx.y];
+  const int64_t n = z.n[blockIdx.y];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
 // ---- host orchestration -----------------------------------------------------
 // Reads back the events of finished stages into the per-stage totals.
 void flush_stage_times(fc_corr* c) {
@@ -542,11 +556,13 @@ void setup_geometry(fc_corr* c) {
 }
 
 void reset_hists(fc_corr* c, hipStream_t s) {
-  fill<int>(c->r_counts.as<int>(), (int64_t)c->nell * c->nrbins, 0, s);
-  fill<double>(c->r_hist.as<double>(), (int64_t)c->nell * c->nrbins, 0.0, s);
-  fill<int>(c->k_counts.as<int>(), (int64_t)c->nell * c->nkbins, 0, s);
-  fill<double>(c->k_hist.as<double>(), (int64_t)c->nell * c->nkbins, 0.0, s);
-  fill<double>(c->zerolag.as<double>(), 1, 0.0, s);
+  const int64_t nr = (int64_t)c->nell * c->nrbins, nk = (int64_t)c->nell * c->nkbins;
+  ZeroSegs z{{c->r_counts.as<uint32_t>(), c->r_hist.as<uint32_t>(), c->k_counts.as<uint32_t>(),
+              c->k_hist.as<uint32_t>(), c->zerolag.as<uint32_t>()},
+             {nr, 2 * nr, nk, 2 * nk, 2}};
+  const int64_t most = std::max<int64_t>(2 * std::max(nr, nk), 2);
+  hipLaunchKernelGGL(k_zero_segs, dim3(grid1d(most, 256, 4096), 5), dim3(256), 0, s, z);
+  FC_LAUNCH_CHECK();
 }
 
 // Orders the correlator's stream after all work queued on the grid's stream.
