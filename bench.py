@@ -224,8 +224,55 @@ def batched_mocks(lib, corr_mod, ctx, reps):
             "mass_assign_ms_per_mock": 1e3 * t_ma / b["mocks"]}
 
 
-SERVE_CFG = dict(rate=20.0, jobs=40, seed=5, box=1000.0, sizes=(128, 256, 384, 512),
+SERVE_CFG = dict(rate=20.0, jobs=200, seed=5, box=1000.0, sizes=(128, 256, 384, 512),
                  npart=(1e5, 1e7), slots=4)
+
+YAMA_CFG = dict(n=512, posmin=500.0, box=2000.0, rmax=200.0, dr=5.0, kmax=0.5, dk=0.005,
+                maxell=4, steps=2)
+
+
+def yamamoto_512(lib, corr_mod, ctx, pos):
+    """The anisotropic stress case: the Yamamoto Correlator (line of sight
+    from an observer at the origin, 15 (l, m) transform pairs for l <= 4) on
+    the headline's 8M points moved to [500, 2500)^3, 512^3 CIC grid, kmax 0.5
+    dk 0.005, rmax 200 dr 5; mixed and FP64 fused forms on the same grid."""
+    c = YAMA_CFG
+    n = c["n"]
+    cell = c["box"] / n
+    posmin = np.full(3, c["posmin"])
+    shifted = np.ascontiguousarray(pos + posmin, np.float64)
+    out = {}
+    for prec in ("mixed", "fp64"):
+        g = lib.DeviceGrid((n, n, n), posmin, cell, prec, ctx)
+        st = lib.MaStats()
+        lib.check(lib.lib().fc_mass_assign(g.handle, lib.dptr(shifted), shifted.shape[0], 3,
+                                           None, 1.0, 3, 0, 0.0, st))
+        g.normalize(st.totw / float(n) ** 3)
+        corr = corr_mod.Correlator((n, n, n), cell, posmin, 3, c["rmax"], c["dr"], c["kmax"],
+                                   c["dk"], c["maxell"], precision=prec)
+        lib.check(lib.lib().fc_corr_autocorrelate(corr._handle, g.handle))  # warm-up
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(c["steps"]):
+            lib.check(lib.lib().fc_corr_autocorrelate(corr._handle, g.handle))
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / c["steps"]
+        corr._fetch()
+        res = {k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v)
+               for k, v in corr.results().items()}
+        out[prec] = (dt, res)
+        del corr, g
+    ref = out["fp64"][1]
+    err = {}
+    for key, name in (("k_hist", "P"), ("r_hist", "xi")):
+        e = multipole_err(out["mixed"][1], ref, key)
+        for q in range(e.shape[0]):
+            err["%s_%d" % (name, 2 * q)] = float(e[q].max())
+    return {"value": 1.0 / out["mixed"][0], "unit": "correlations/s",
+            "ms_per_correlation": 1e3 * out["mixed"][0],
+            "fp64_fused": 1.0 / out["fp64"][0], "max_floor_rel_err_vs_fp64": err,
+            "config": "Correlator 512^3 l<=4 (15 Y_lm transforms), observer at the origin, "
+                      "headline points in [500, 2500)^3, CIC, kmax 0.5 dk 0.005, rmax 200 dr 5"}
 
 
 def serving(lib, corr_mod):
@@ -452,6 +499,8 @@ def main():
                          "numpy/scipy oracle (scipy.fft workers=%d), %.1f s" % (threads, dt),
                "gpu_fp64_vs_cpu_max_floor_rel_err": e_cpu}
 
+    log("Yamamoto correlator at 512^3 (anisotropic stress case)")
+    yama_report = yamamoto_512(lib, corr_mod, ctx, pos)
     log("batched mocks: 64 x 256^3 TSC, mixed, HIP graph")
     batch_report = batched_mocks(lib, corr_mod, ctx, 5)
     log("serving: Poisson stream of mixed-size jobs")
@@ -503,6 +552,7 @@ def main():
         "mixed_over_fp64_fused": value / rate(forms["fp64_fused"]),
         "fp64_plain_vs_fused_max_floor_rel_err": plain_vs_fused,
         "mass_assign_ms_two_grids": fm["ma_ms"],
+        "yamamoto_512": yama_report,
         "batched_256": batch_report,
         "serving": serve_report,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
