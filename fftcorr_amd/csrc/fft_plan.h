@@ -197,7 +197,9 @@ void launch_rows_r2c(const FftPlan& p, const Tin* in, int64_t ldi, cplx<T>* out,
     const FftPlan& pt = std::is_same<T, A>::value ? p : *pa;
     const cplx<A>* twMa = pt.tw2M.as<cplx<A>>();
     const cplx<A>* twNa = pt.tw2N.as<cplx<A>>();
-    int trc = tr;
+    // Wider arithmetic: at most 8 rows per workgroup (smaller, more
+    // independent workgroups measured faster than the FP32 form's tile).
+    int trc = std::is_same<T, A>::value ? tr : (tr < 8 ? tr : 8);
     while (trc > 1 && (size_t)trc * row_pad_len(M) * cs > kLdsBudget) trc /= 2;
 #define FC_R2C_RC_CASE(Mv)                                                                   \
   case Mv: {                                                                                 \
