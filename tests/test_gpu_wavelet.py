@@ -4,8 +4,9 @@ sample table d12.h) with the table registered by the caller.
 The grid is checked against the oracle (the table-driven stencil, itself
 pinned against the reference's own code) at 1e-13 relative; the full CLI at
 the configuration of the reference's golden outputs (256^3, l <= 2, r <= 200
-by 10, k <= 0.03 by 0.01, D12, DD - RR on the north catalogues) reproduces
-test_data/north/nonperiodic/corrDD.dat.out at the printed precision."""
+by 10, k <= 0.03 by 0.01, D12; DD - RR and RR on the north and south
+catalogues) reproduces test_data/{north,south}/nonperiodic/corr{DD,RR}.dat.out
+at the printed precision."""
 import ctypes
 import os
 
@@ -79,13 +80,17 @@ def test_d12_needs_table_and_eight_cells(table):
         cli.run(["-r", "50", "-d", "2", "-in", os.path.join(GOLDEN, "north", "corrDD.dat")])
 
 
-def test_cli_d12_reproduces_reference_golden(table, tmp_path):
-    f1 = os.path.join(GOLDEN, "north", "corrDD.dat")
-    f2 = os.path.join(GOLDEN, "north", "corrRR.dat")
-    text = cli.run(["-n", "256", "-ell", "2", "-r", "200", "-dr", "10", "-in", f1, "-in2", f2,
-                    "-wisdom", str(tmp_path / "w.txt")])
+@pytest.mark.parametrize("hemi,kind", [("north", "DD"), ("north", "RR"), ("south", "DD"),
+                                       ("south", "RR")])
+def test_cli_d12_reproduces_reference_golden(table, tmp_path, hemi, kind):
+    # DD runs take the randoms as the second file (D - R), RR runs the
+    # randoms alone, as the golden outputs were made.
+    f1 = os.path.join(GOLDEN, hemi, "corr%s.dat" % kind)
+    extra = ["-in2", os.path.join(GOLDEN, hemi, "corrRR.dat")] if kind == "DD" else []
+    text = cli.run(["-n", "256", "-ell", "2", "-r", "200", "-dr", "10", "-in", f1] + extra +
+                   ["-wisdom", str(tmp_path / "w.txt")])
     assert "# Using D12 wavelet" in text
-    golden = os.path.join(GOLDEN, "north", "nonperiodic", "corrDD.dat.out")
+    golden = os.path.join(GOLDEN, hemi, "nonperiodic", "corr%s.dat.out" % kind)
     gold = np.loadtxt(golden)
     got = _rows(text)
     assert got.shape == gold.shape
