@@ -140,3 +140,114 @@ def test_interlace_recovers_band_limited_field():
     # A flipped phase sign would not reproduce F_a.
     bad = 0.5 * (fa + fb * np.conj(O.interlace(np.zeros_like(fa), np.ones_like(fa), n) * 2))
     assert np.abs(bad - fa).max() > 1e-3 * np.abs(fa).max()
+
+
+# ---------------------------------------------------------------------------
+# Periodic multipoles l = 0, 2, 4 against the direct-definition estimators of
+# tests/brute_force.py (Legendre polynomials, plain DFT sums, pair products):
+# pins the (2l+1) L_l normalisation, the line of sight (z), the cell^3/N and
+# 1/N^2 factors and the window division independently of make_ylm and of the
+# oracle's coefficients (cc/correlate/correlator.h:296-357).
+import brute_force as BF  # noqa: E402
+
+_PERIODIC_PIN_CASES = [
+    # shape, cell, window (oracle code, brute-force name), rmax, dr, kmax, dk
+    ((8, 8, 8), 10.0, (0, None), 30.0, 10.0, 0.33, 0.037),
+    ((6, 7, 9), 7.0, (3, "cic"), 21.0, 7.0, 0.47, 0.061),
+    ((8, 6, 10), 5.0, (1, "tsc"), 15.0, 5.0, 0.6, 0.087),
+]
+
+
+def _anisotropic_field(shape, seed):
+    """A random field with different correlation lengths along x, y and z,
+    so every multipole and every choice of line of sight differ."""
+    rng = np.random.default_rng(seed)
+    a = rng.normal(size=shape)
+    F = np.fft.rfftn(a)
+    k = [np.fft.fftfreq(n) for n in shape[:2]] + [np.fft.rfftfreq(shape[2])]
+    kk = (0.6 * k[0][:, None, None] ** 2 + 1.0 * k[1][None, :, None] ** 2
+          + 3.0 * k[2][None, None, :] ** 2)
+    return np.fft.irfftn(F * np.exp(-kk / 0.05), s=shape, axes=(0, 1, 2)) + 0.1 * a
+
+
+@pytest.mark.parametrize("case", _PERIODIC_PIN_CASES)
+def test_periodic_multipoles_match_legendre_brute_force(case):
+    shape, cell, (wcode, wname), rmax, dr, kmax, dk = case
+    delta = _anisotropic_field(shape, 40 + shape[2])
+    c = O.PeriodicCorrelator(shape, cell, wcode, rmax, dr, kmax, dk, 4)
+    res = c.autocorrelate(delta)
+    kc, ks = BF.power_multipoles(delta, cell, kmax, dk, 4, window=wname)
+    rc, rsum = BF.correlation_multipoles(delta, cell, rmax, dr, 4)
+    np.testing.assert_array_equal(res["k_counts"], kc)
+    np.testing.assert_array_equal(res["r_counts"], rc)
+    # Every multipole is far from zero somewhere, so a wrong coefficient or
+    # axis cannot hide inside the tolerance.
+    for q in range(3):
+        assert np.abs(ks[q]).max() > 1e-3 * np.abs(ks[0]).max()
+        assert np.abs(rsum[q]).max() > 1e-3 * np.abs(rsum[0]).max()
+    np.testing.assert_allclose(res["k_hist"], ks, rtol=1e-10, atol=1e-12 * np.abs(ks).max())
+    np.testing.assert_allclose(res["r_hist"], rsum, rtol=1e-10,
+                               atol=1e-12 * np.abs(rsum).max())
+
+
+def test_periodic_cross_multipoles_match_brute_force():
+    shape, cell = (8, 8, 8), 10.0
+    d1 = _anisotropic_field(shape, 50)
+    d2 = 0.5 * d1 + _anisotropic_field(shape, 51)
+    c = O.PeriodicCorrelator(shape, cell, 0, 30.0, 10.0, 0.33, 0.037, 4)
+    res = c.cross_correlate(d1, d2)
+    kc, ks = BF.power_multipoles(d1, cell, 0.33, 0.037, 4, delta2=d2)
+    rc, rsum = BF.correlation_multipoles(d1, cell, 30.0, 10.0, 4, delta2=d2)
+    np.testing.assert_array_equal(res["k_counts"], kc)
+    np.testing.assert_allclose(res["k_hist"], ks, rtol=1e-10, atol=1e-12 * np.abs(ks).max())
+    np.testing.assert_allclose(res["r_hist"], rsum, rtol=1e-10,
+                               atol=1e-12 * np.abs(rsum).max())
+
+
+def test_periodic_multipole_pin_rejects_plausible_slips():
+    """The brute force tells apart the slips a restatement could make: a
+    sqrt(4 pi / (2l+1)) coefficient and a line of sight along x."""
+    shape, cell = (8, 8, 8), 10.0
+    delta = _anisotropic_field(shape, 48)
+    kc, ks = BF.power_multipoles(delta, cell, 0.33, 0.037, 4)
+    c = O.PeriodicCorrelator(shape, cell, 0, 30.0, 10.0, 0.33, 0.037, 4)
+    good = c.autocorrelate(delta)["k_hist"]
+    # LOS along x: the field transposed so its z axis becomes x.
+    swapped = c.autocorrelate(np.ascontiguousarray(np.transpose(delta, (2, 1, 0))))["k_hist"]
+    for q in (1, 2):
+        ell = 2 * q
+        wrong_coeff = good[q] / (2 * ell + 1)  # sqrt(4pi/(2l+1)) Y_l0 = L_l
+        assert not np.allclose(wrong_coeff, ks[q], rtol=1e-3)
+        assert not np.allclose(swapped[q], ks[q], rtol=1e-3)
+
+
+# ---------------------------------------------------------------------------
+# The far-observer limit (cc/correlate/correlator.h:388-394): with the observer
+# 1e9 cells away along -z, every cell's line of sight is +z to 1e-8, and the
+# Yamamoto Correlator must reproduce the periodic multipoles after its own
+# normalisations: k histograms carry 4 pi without cell^3/N (periodic / Yamamoto
+# = cell^3/N), r histograms 4 pi/N instead of 1/N^2 (periodic / Yamamoto = 1/N).
+@pytest.mark.parametrize("shape,cell,wcode", [((8, 8, 8), 10.0, 0), ((6, 8, 7), 6.0, 3)])
+def test_far_observer_yamamoto_equals_periodic(shape, cell, wcode):
+    delta = _anisotropic_field(shape, 60 + shape[1])
+    rmax, dr, kmax, dk = 3 * cell, cell, 0.4 * 10.0 / cell, 0.05 * 10.0 / cell
+    per = O.PeriodicCorrelator(shape, cell, wcode, rmax, dr, kmax, dk, 4).autocorrelate(delta)
+    observer = [shape[0] / 2.0, shape[1] / 2.0, -1e9]
+    yam = O.Correlator(shape, cell, [0, 0, 0], wcode, rmax, dr, kmax, dk, 4,
+                       observer=observer).autocorrelate(delta)
+    N = delta.size
+    np.testing.assert_array_equal(per["k_counts"], yam["k_counts"])
+    np.testing.assert_array_equal(per["r_counts"], yam["r_counts"])
+    k_scale = cell ** 3 / N
+    for q in range(3):
+        ref_k = per["k_hist"][q]
+        ref_r = per["r_hist"][q]
+        np.testing.assert_allclose(yam["k_hist"][q] * k_scale, ref_k, rtol=1e-6,
+                                   atol=1e-7 * np.abs(per["k_hist"][0]).max())
+        np.testing.assert_allclose(yam["r_hist"][q] / N, ref_r, rtol=1e-6,
+                                   atol=1e-7 * np.abs(per["r_hist"][0]).max())
+    # An observer along -x instead gives a line of sight along x: the l = 2
+    # multipoles then differ from the periodic (LOS z) ones.
+    yx = O.Correlator(shape, cell, [0, 0, 0], wcode, rmax, dr, kmax, dk, 4,
+                      observer=[-1e9, shape[1] / 2.0, shape[2] / 2.0]).autocorrelate(delta)
+    assert not np.allclose(yx["k_hist"][1] * k_scale, per["k_hist"][1], rtol=1e-3)
