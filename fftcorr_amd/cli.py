@@ -110,6 +110,21 @@ def hist_lines(bins, counts, hist, prefix, normalize):
     return out
 
 
+def grid_geometry(posmin, posmax, ngrid, cell, sep, periodic):
+    """Grid corner and cell of the CLI (cc/fftcorr.cc:335-346): a
+    non-periodic box is padded by sep / 1.5 on every side; the cell is the
+    requested one or the smallest that covers the box, whichever is larger."""
+    posmin = np.asarray(posmin, np.float64)
+    posmax = np.asarray(posmax, np.float64)
+    if not periodic:
+        pad = sep / 1.5
+        posmin = posmin - pad
+        posmax = posmax + pad
+    for i in range(3):
+        cell = max(cell, (posmax[i] - posmin[i]) / ngrid[i])
+    return posmin, cell
+
+
 def run(argv, stdout=None):
     o = parse(argv)
     from fftcorr_amd.particle_mesh import wavelet_table_set
@@ -125,18 +140,14 @@ def run(argv, stdout=None):
         fftw.import_wisdom_from_file(o["wisdom"])
     posmin, posmax = read_header(o["infile"])
     sep = o["sep"]
-    if not o["periodic"]:
-        pad = sep / 1.5
-        posmin = posmin - pad
-        posmax = posmax + pad
     ngrid = o["ngrid"]
-    cell = o["cell"]
-    for i in range(3):
-        cell = max(cell, (posmax[i] - posmin[i]) / ngrid[i])
+    posmin, cell = grid_geometry(posmin, posmax, ngrid, o["cell"], sep, o["periodic"])
     w.append("# Adopted cell_size=%f for ngrid=[%d, %d, %d]" % (cell, *ngrid))
     w.append("# Adopted boxsize: %6.1f %6.1f %6.1f" % tuple(cell * n for n in ngrid))
-    gmax = [posmin[i] + cell * ngrid[i] for i in range(3)]
-    grid = ConfigSpaceGrid(ngrid, posmin, gmax, cell_size=cell, precision=o["precision"])
+    # The adopted cell is taken as given, as the reference's CLI builds its
+    # grid from (ngrid, posmin, cell): recomputing it from posmax could round
+    # one ulp above the adopted value.
+    grid = ConfigSpaceGrid.from_cell(ngrid, posmin, cell, precision=o["precision"])
     ma = MassAssignor(grid, o["scheme"], o["periodic"], 1000000)
     for f in [o["infile"]] + ([o["infile2"]] if o["infile2"] else []):
         w.append("# Reading from file named %s" % f)

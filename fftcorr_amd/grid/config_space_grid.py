@@ -1,88 +1,137 @@
-"""ConfigSpaceGrid (fftcorr/grid/config_space_grid.pyx:11-207,
-cc/grid/config_space_grid.h:10-56).
+"""ConfigSpaceGrid: the density grid of the correlation path.
 
-The grid keeps the reference's host view: ``.data`` is a writable float64
-[n0, n1, n2] numpy array.  Alongside it lives a device copy in HBM that mass
-assignment and the correlators use directly.  Which copy is current is
-tracked: device work (mass assignment, in-place scalar ops) marks the host
-copy stale and ``.data`` downloads it; handing out ``.data`` marks the device
-copy stale (the caller may write through the view) and the next device use
-uploads it.
+Interface of fftcorr/grid/config_space_grid.pyx:11-207 (Python) over
+cc/grid/config_space_grid.h:10-56 (C++): a regular [n0, n1, n2] grid of cubic
+cells of side ``cell_size`` whose corner sits at ``posmin``.
+
+Geometry contract (what the reference's constructor guarantees):
+  * shape: 3 positive ints; posmin < posmax elementwise; cell_size > 0 when
+    given; padding and extra_pad >= 0.  A violation raises ValueError.
+  * extra_pad widens the box by that much on every side before adoption.
+  * Adoption: the cell is the smallest that covers the box on every axis,
+    max_i (posmax_i - posmin_i) / shape_i, or the requested cell_size if it is
+    at least that; posmax becomes posmin + shape * cell.
+  * posmin and posmax are exposed read-only; padding reports
+    padding + extra_pad.
+
+Storage: ``.data`` is a writable float64 host array, as in the reference,
+and the grid also keeps a device copy in HBM that mass assignment and the
+correlators work on.  Coherence rule, so that ``.data`` behaves as a live
+view of the grid:
+  * device work marks the host copy stale; reading ``.data`` downloads it;
+  * handing out ``.data`` marks the device copy stale, and the next device use
+    uploads it;
+  * while any reference to the host array (or a view of it) is held outside
+    this object, the host copy is primary: every device use uploads first, and
+    every device modification is downloaded at once, so a kept view both sees
+    device results and has its writes picked up.  Whether such a reference
+    exists is read from the array's reference count.
 """
 
+import logging
 import math
+import sys
 
 import numpy as np
 
 from fftcorr_amd import _lib
 
+_log = logging.getLogger("fftcorr_amd.grid")
+
+# Slack of the coverage check, in units of the box coordinates' ulp: a cell
+# computed elsewhere as max((posmax - posmin) / shape) (the CLI's adoption)
+# may fall short of this module's by the rounding of that difference.
+_COVER_ULPS = 8 * np.finfo(np.float64).eps
+
+
+def _triple(name, value, dtype):
+    arr = np.array(value, dtype=dtype)
+    if arr.shape != (3,):
+        raise ValueError("%s must hold 3 values, got shape %s" % (name, arr.shape))
+    return arr
+
 
 class ConfigSpaceGrid:
     def __init__(self, shape, posmin, posmax, cell_size=None, padding=0, extra_pad=0,
                  metadata=None, precision="fp64", device=None):
-        shape = np.ascontiguousarray(shape, dtype=np.intc)
-        posmin = np.asarray(posmin, dtype=np.double).copy(order="C")
-        posmax = np.asarray(posmax, dtype=np.double).copy(order="C")
-        # Checks and messages of config_space_grid.pyx:27-62.
-        if shape.shape != (3,):
-            raise ValueError("Expected shape to have shape (3,), got: {}".format(shape.shape))
-        if np.any(shape <= 0):
-            raise ValueError("Expected shape to be positive, got: {}".format(shape))
-        if posmin.shape != (3,):
-            raise ValueError("Expected posmin to have shape (3,), got: {}".format(posmin.shape))
-        if posmax.shape != (3,):
-            raise ValueError("Expected posmax to have shape (3,), got: {}".format(posmax.shape))
-        if np.any(posmax <= posmin):
-            raise ValueError("Expected posmin < posmax, got posmin={}, posmax={}".format(
-                posmin, posmax))
-        if cell_size is not None and cell_size <= 0:
-            raise ValueError("Expected cell_size to be positive, got: {}".format(cell_size))
-        if padding < 0:
-            raise ValueError("Expected padding > 0, got {}".format(padding))
-        if extra_pad < 0:
-            raise ValueError("Expected extra_pad > 0, got {}".format(extra_pad))
-        if extra_pad:
-            print("Padding boundaries by {:.6g}".format(extra_pad))
-            posmin -= extra_pad
-            posmax += extra_pad
-        # Cell-size adoption, config_space_grid.pyx:68-78.
-        minimum_cell_size = np.amax((posmax - posmin) / shape)
-        if cell_size is not None:
-            print("Requested cell size {:.6g}".format(cell_size))
-            if cell_size < minimum_cell_size:
-                raise ValueError("Requested cell size too small to cover entire region. "
-                                 "Need at least {:.6g}".format(minimum_cell_size))
+        n = _triple("shape", shape, np.intc)
+        if (n <= 0).any():
+            raise ValueError("shape entries must be positive, got %s" % (n.tolist(),))
+        lo = _triple("posmin", posmin, np.float64)
+        hi = _triple("posmax", posmax, np.float64)
+        if not (lo < hi).all():
+            raise ValueError("need posmin < posmax on every axis, got posmin=%s posmax=%s"
+                             % (lo.tolist(), hi.tolist()))
+        if padding < 0 or extra_pad < 0:
+            raise ValueError("padding and extra_pad must be non-negative, got %r and %r"
+                             % (padding, extra_pad))
+        if cell_size is not None and not cell_size > 0:
+            raise ValueError("cell_size must be positive, got %r" % (cell_size,))
+        lo = lo - extra_pad
+        hi = hi + extra_pad
+        cover = float(np.max((hi - lo) / n))
+        if cell_size is None:
+            cell = cover
         else:
-            cell_size = minimum_cell_size
-        posmax = posmin + shape * cell_size
-        print("Adopting:")
-        print("  shape = [" + ", ".join(["{}".format(x) for x in shape]) + "]")
-        print("  cell_size = {:.6g}".format(cell_size))
-        print("  posmin = [" + ", ".join(["{:.6g}".format(x) for x in posmin]) + "]")
-        print("  posmax = [" + ", ".join(["{:.6g}".format(x) for x in posmax]) + "]")
-        for arr in (posmin, posmax):
-            arr.setflags(write=False)
-        self._shape = tuple(int(s) for s in shape)
-        self._posmin = posmin
-        self._posmax = posmax
-        self._cell_size = float(cell_size)
-        self._padding = padding + extra_pad
-        self._metadata = {}
-        if metadata:
-            self._metadata.update(metadata)
+            cell = float(cell_size)
+            reach = lo + n * cell
+            slack = _COVER_ULPS * np.maximum(np.maximum(np.abs(lo), np.abs(hi)), n * cell)
+            if (reach < hi - slack).any():
+                raise ValueError("cell_size %.6g does not cover the box: it needs at least %.6g"
+                                 % (cell, cover))
+        self._setup(n, lo, cell, padding + extra_pad, metadata, precision, device)
+
+    @classmethod
+    def from_cell(cls, shape, posmin, cell_size, padding=0, metadata=None, precision="fp64",
+                  device=None):
+        """A grid of ``shape`` cells of side ``cell_size`` from ``posmin``,
+        taken as given (no coverage check): the construction of the
+        reference's CLI, which builds its grid from (ngrid, posmin, cell)
+        (cc/fftcorr.cc:335-346, cc/grid/config_space_grid.h:12-16)."""
+        n = _triple("shape", shape, np.intc)
+        if (n <= 0).any():
+            raise ValueError("shape entries must be positive, got %s" % (n.tolist(),))
+        if not cell_size > 0:
+            raise ValueError("cell_size must be positive, got %r" % (cell_size,))
+        self = cls.__new__(cls)
+        self._setup(n, _triple("posmin", posmin, np.float64), float(cell_size), padding,
+                    metadata, precision, device)
+        return self
+
+    def _setup(self, n, lo, cell, padding, metadata, precision, device):
+        hi = lo + n * cell
+        lo.setflags(write=False)
+        hi.setflags(write=False)
+        self._shape = tuple(int(s) for s in n)
+        self._posmin = lo
+        self._posmax = hi
+        self._cell_size = cell
+        self._padding = padding
+        self._metadata = dict(metadata) if metadata else {}
         self._precision = _lib.grid_precision_code(precision)
         self._device = device
         self._host = np.zeros(self._shape, dtype=np.float64)
+        self._host_refs = sys.getrefcount(self._host)
         self._dev = None
         self._host_valid = True
         self._dev_valid = False
+        _log.info("grid %s, cell %.6g, posmin %s, posmax %s", self._shape, cell, lo.tolist(),
+                  hi.tolist())
 
     # ---- host / device coherence -------------------------------------------
+    def _view_held(self):
+        """True while something outside this object references the host
+        array (directly or through a numpy view, whose base it is)."""
+        return sys.getrefcount(self._host) > self._host_refs
+
     def device_grid(self):
         """The device copy (an _lib.DeviceGrid), uploaded if stale."""
         if self._dev is None:
             self._dev = _lib.DeviceGrid(self._shape, self._posmin, self._cell_size,
                                         self._precision, _lib.context(self._device))
+            self._dev_valid = False
+        if self._view_held():
+            self._sync_host()
             self._dev_valid = False
         if not self._dev_valid:
             self._dev.upload(self._host)
@@ -90,8 +139,11 @@ class ConfigSpaceGrid:
         return self._dev
 
     def _device_modified(self):
+        """Called after device work changed the grid."""
         self._dev_valid = True
         self._host_valid = False
+        if self._view_held():
+            self._sync_host()
 
     def _sync_host(self):
         if not self._host_valid:
@@ -100,21 +152,20 @@ class ConfigSpaceGrid:
 
     # ---- reference API -----------------------------------------------------
     def to_grid_coords(self, survey_coords, periodic_wrap=False):
-        """cc/grid/config_space_grid.h:31-46 for one point."""
-        survey_coords = np.asarray(survey_coords, dtype=np.float64)
-        out = np.empty_like(survey_coords)
-        for i in range(3):
-            x = (survey_coords[i] - self._posmin[i]) / self._cell_size
-            xmax = float(self._shape[i])
-            if x < 0 or x >= xmax:
-                if not periodic_wrap:
-                    raise ValueError(f"Coordinates {survey_coords} out of bounds")
-                x = math.fmod(x, xmax)
-                if x < 0:
-                    x += xmax
-                if x >= xmax:
-                    x = 0.0
-            out[i] = x
+        """Survey coordinates of one point to grid units, (p - posmin) / cell
+        per axis (cc/grid/config_space_grid.h:31-46).  Out of range: wrapped
+        into [0, n) when periodic_wrap, else ValueError."""
+        p = np.asarray(survey_coords, dtype=np.float64)
+        out = (p - self._posmin) / self._cell_size
+        for i, ni in enumerate(self._shape):
+            if 0.0 <= out[i] < ni:
+                continue
+            if not periodic_wrap:
+                raise ValueError("point %s lies outside the grid" % (p.tolist(),))
+            w = math.fmod(out[i], ni)
+            if w < 0:
+                w += ni
+            out[i] = 0.0 if w >= ni else w  # -tiny + n rounds to n: that is cell 0
         return out
 
     @property
@@ -152,7 +203,7 @@ class ConfigSpaceGrid:
         return "fp64" if self._precision == _lib.FC_FP64 else "fp32"
 
     def clear(self):
-        self._host[...] = 0.0
+        self._host.fill(0.0)
         self._host_valid = True
         if self._dev is not None:
             self._dev.clear()
@@ -177,27 +228,40 @@ class ConfigSpaceGrid:
         self._device_modified()
         return mean
 
+    # ---- asdf files (the reference's on-disk format) --------------------------
+    def _header(self):
+        return {"shape": list(self._shape), "posmin": self._posmin.tolist(),
+                "posmax": self._posmax.tolist(), "cell_size": self._cell_size,
+                "padding": self._padding}
+
     def write(self, filename, dtype=np.float32):
-        import asdf  # the reference's format; not bundled here
-        tree = {"header": {"shape": self.shape, "posmin": self.posmin, "posmax": self.posmax,
-                           "cell_size": self.cell_size, "padding": self.padding},
-                "data": self.data.astype(dtype), "metadata": self.metadata}
-        with asdf.AsdfFile(tree) as af:
-            af.write_to(filename)
+        """Writes header, data (as dtype) and metadata to an asdf file; needs
+        the asdf package."""
+        import asdf
+        tree = {"header": self._header(), "data": np.asarray(self.data, dtype=dtype),
+                "metadata": dict(self._metadata)}
+        asdf.AsdfFile(tree).write_to(filename)
 
     @classmethod
-    def read(cls, filename):
+    def read(cls, filename, precision="fp64", device=None):
+        """Reads a grid written by write() (or by the reference)."""
         import asdf
-        with asdf.open(filename) as af:
-            header = af.tree["header"]
-            grid = cls(shape=header["shape"], posmin=header["posmin"], posmax=header["posmax"],
-                       padding=header["padding"], metadata=af.tree.get("metadata"))
-            assert np.allclose(grid.cell_size, header["cell_size"])
-            np.copyto(grid.data, af.tree["data"])
+        with asdf.open(filename) as f:
+            h = f.tree["header"]
+            grid = cls.from_cell(h["shape"], h["posmin"], float(h["cell_size"]),
+                                 padding=h.get("padding", 0), metadata=f.tree.get("metadata"),
+                                 precision=precision, device=device)
+            if not np.allclose(grid.posmax, h["posmax"]):
+                raise ValueError("inconsistent header in %s: posmax %s, expected %s"
+                                 % (filename, h["posmax"], grid.posmax.tolist()))
+            np.copyto(grid.data, np.asarray(f.tree["data"]))
         return grid
 
+    # ---- in-place arithmetic ---------------------------------------------------
     def _inplace(self, other, op, dev_op):
-        if np.isscalar(other) and self._dev_valid and not self._host_valid and dev_op:
+        device_current = self._dev is not None and self._dev_valid and not self._host_valid
+        if np.isscalar(other) and dev_op is not None and device_current \
+                and not self._view_held():
             dev_op(float(other))
             self._device_modified()
         else:
