@@ -18,14 +18,16 @@ from fftcorr_amd._table import make_table
 
 
 class _MemberGrid(_lib.DeviceGrid):
-    """A batch member's grid: an fc_grid owned by the batch."""
+    """A batch member's grid: an fc_grid owned by the batch.  It holds no
+    reference to the batch (so ``del batch`` frees the batch's HBM at once);
+    when the batch closes, the member's handle is cleared and later calls
+    through it fail with an error instead of touching freed memory."""
 
-    def __init__(self, batch, handle, shape, precision):
-        self.ctx = batch._ctx
+    def __init__(self, ctx, handle, shape, precision):
+        self.ctx = ctx
         self.shape = shape
         self.precision = precision
         self.handle = handle
-        self._batch = batch  # keeps the batch (and its memory) alive
 
     def close(self):
         self.handle = None
@@ -67,19 +69,38 @@ class PeriodicBatch:
         for i in range(self.nbatch):
             g = ctypes.c_void_p()
             _lib.check(_lib.lib().fc_batch_grid(h, i, ctypes.byref(g)))
-            self._grids.append(_MemberGrid(self, g, self._shape, gprec))
+            self._grids.append(_MemberGrid(self._ctx, g, self._shape, gprec))
+
+    def close(self):
+        """Destroys the batch and frees its device memory now; the member
+        grids become invalid."""
+        for g in getattr(self, "_grids", ()):
+            g.close()
+        h = getattr(self, "_handle", None)
+        self._handle = None
+        if h:
+            _lib.check(_lib.lib().fc_batch_destroy(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
-        h = getattr(self, "_handle", None)
-        if h:
-            try:
-                _lib.lib().fc_batch_destroy(h)
-            except Exception:
-                pass
-            self._handle = None
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _live(self):
+        if not self._handle:
+            raise RuntimeError("the batch has been closed")
+        return self._handle
 
     def grid(self, i):
         """Member i's device grid (fill with fc_mass_assign / upload, then normalize)."""
+        self._live()
         return self._grids[i]
 
     def assign(self, i, positions, weight=None, scheme=1, periodic_wrap=True, shift=0.0):
@@ -89,7 +110,7 @@ class PeriodicBatch:
         pos = np.ascontiguousarray(positions, np.float64)
         if pos.ndim != 2 or pos.shape[1] not in (3, 4):
             raise ValueError("positions must be (n, 3) or (n, 4)")
-        g = self._grids[i]
+        g = self.grid(i)
         g.clear()
         st = _lib.MaStats()
         w = None if weight is None or np.isscalar(weight) else np.ascontiguousarray(weight,
@@ -105,12 +126,12 @@ class PeriodicBatch:
     def autocorrelate(self, n=None):
         """Correlates members 0..n-1 (all by default)."""
         n = self.nbatch if n is None else int(n)
-        _lib.check(_lib.lib().fc_batch_autocorrelate(self._handle, n))
+        _lib.check(_lib.lib().fc_batch_autocorrelate(self._live(), n))
 
     @property
     def last_time_ms(self):
         t = ctypes.c_double()
-        _lib.check(_lib.lib().fc_batch_last_time_ms(self._handle, ctypes.byref(t)))
+        _lib.check(_lib.lib().fc_batch_last_time_ms(self._live(), ctypes.byref(t)))
         return t.value
 
     def results(self, i):
@@ -120,7 +141,7 @@ class PeriodicBatch:
         kh = np.zeros((self._ne, self._nk))
         kc = np.zeros((self._ne, self._nk), np.intc)
         z = ctypes.c_double()
-        _lib.check(_lib.lib().fc_batch_results(self._handle, int(i), _lib.dptr(rh), _lib.iptr(rc),
+        _lib.check(_lib.lib().fc_batch_results(self._live(), int(i), _lib.dptr(rh), _lib.iptr(rc),
                                                _lib.dptr(kh), _lib.iptr(kc), ctypes.byref(z)))
         return {"r": self._r.copy(), "r_counts": rc, "r_hist": rh, "k": self._k.copy(),
                 "k_counts": kc, "k_hist": kh, "zerolag": z.value}
