@@ -3,7 +3,10 @@
 Each .hip source compiles to an object in parallel, then one shared library
 is linked in-tree at fftcorr_amd/_lib/libfftcorr_hip.so.
 
-    python -m fftcorr_amd.build [--jobs N]
+    python -m fftcorr_amd.build [--jobs N] [--incremental]
+
+--incremental recompiles only sources older than their object or than any
+header of csrc/ or include/ (a header change rebuilds everything).
 """
 
 import argparse
@@ -32,8 +35,23 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
+def _obj(src):
+    return os.path.join(OBJDIR, os.path.basename(src) + ".o")
+
+
+def _stale(src):
+    obj = _obj(src)
+    if not os.path.exists(obj):
+        return True
+    headers = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    inc = os.path.join(ROOT, "include")
+    headers += [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith(".h")]
+    newest = max([os.path.getmtime(src)] + [os.path.getmtime(h) for h in headers])
+    return os.path.getmtime(obj) < newest
+
+
 def _compile(src):
-    obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
+    obj = _obj(src)
     cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -41,12 +59,14 @@ def _compile(src):
     return obj
 
 
-def build(jobs=None, verbose=True):
+def build(jobs=None, verbose=True, incremental=False):
     os.makedirs(OBJDIR, exist_ok=True)
     srcs = sources()
-    jobs = jobs or min(len(srcs), os.cpu_count() or 4, 8)
+    todo = [s for s in srcs if _stale(s)] if incremental else srcs
+    jobs = jobs or min(max(len(todo), 1), os.cpu_count() or 4, 8)
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        list(ex.map(_compile, todo))
+    objs = [_obj(s) for s in srcs]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -61,9 +81,10 @@ def build(jobs=None, verbose=True):
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--incremental", action="store_true")
     a = ap.parse_args()
     try:
-        build(a.jobs)
+        build(a.jobs, incremental=a.incremental)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

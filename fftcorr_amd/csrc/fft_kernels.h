@@ -674,9 +674,10 @@ __global__ void __launch_bounds__(1024)
 // columns):
 //   input  element (a, e, g, b) at a*inA  + e*inL  + g*inG  + b,
 //   output element (a, o, g, b) at a*outA + o*outL + g*outG + b,
-// with o in [0, nout).  nout == L writes every output (o = e); nout != L
-// writes only outputs e = (o - off) mod L, o < nout (the pruned inverse: only
-// the lags a separation subgrid needs).  In place when in == out.
+// with o in [0, nout).  off == 0 writes outputs o = e (nout == L: every
+// output); off != 0 writes outputs o < nout holding element e = (o - off)
+// mod L (the pruned inverse: only the lags a separation subgrid needs, off =
+// nout / 2; nout may equal or exceed L).  In place when in == out.
 // grid = (G * tilesB, A), tilesB = ceil(B / tc); block = tc * (L / E);
 // LDS = L * tc complex.
 // The input may be stored in another precision Tin (converted on load).
@@ -719,7 +720,7 @@ __global__ void __launch_bounds__(1024)
   fft_lds<T, L, DIR>(lds, tw, tid % tc, tid / tc, LayColFast{tc});
   for (int idx = tid; idx < nout * tc; idx += nthr) {
     const int o = idx / tc, c = idx - o * tc;
-    const int e = nout == L ? o : ((o - off) % L + L) % L;
+    const int e = off == 0 ? o : ((o - off) % L + L) % L;
     if (c < ncol) dst[o * outL + c] = lds[e * tc + c];
   }
 }
@@ -744,7 +745,7 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   for (int o = threadIdx.x; o < nout; o += blockDim.x) {
-    const int k = nout == L ? o : ((o - off) % L + L) % L;
+    const int k = off == 0 ? o : ((o - off) % L + L) % L;
     T sx = 0, sy = 0;
     int idx = 0;
     for (int e = 0; e < L; ++e) {
