@@ -126,6 +126,9 @@ SIGNATURES = {
     "fc_corr_stage_times": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "fc_hist_accumulate": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                           _dp, _dp, _i64, _ip, _dp]),
+    "fc_hist_accumulate_device_noalloc": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
+                                                         ctypes.c_int, _vp, _vp, _i64, _vp,
+                                                         _vp]),
     "fc_fft_r2c": (ctypes.c_int, [_vp, _ip, ctypes.c_int, _dp, _dp]),
     "fc_fft_c2r": (ctypes.c_int, [_vp, _ip, ctypes.c_int, _dp, _dp]),
     "fc_plan_cache_export": (ctypes.c_int, [ctypes.c_char_p]),

@@ -28,6 +28,10 @@ inline void fail(int status, const std::string& msg) { throw Error(status, msg);
 
 void set_last_error(const std::string& msg);
 
+// Records the device address of the no-allocation histogram scratch array on
+// `device` (the current device); called once per device at context creation.
+void noalloc_scratch_init(int device);
+
 #define FC_HIP(expr)                                                              \
   do {                                                                            \
     hipError_t _e = (expr);                                                       \

@@ -50,6 +50,7 @@ int fc_ctx_create(int device, fc_ctx** out) {
     hipDeviceProp_t prop;
     FC_HIP(hipGetDeviceProperties(&prop, device));
     c->arch_cus = prop.multiProcessorCount;
+    fc::noalloc_scratch_init(device);  // once per device, with the device current
     *out = c.release();
   });
 }

@@ -240,6 +240,20 @@ int fc_batch_last_time_ms(fc_batch* batch, double* ms);
 int fc_hist_accumulate(fc_ctx* ctx, double minval, double binsize, int nbins,
                        const double* x, const double* y, int64_t n, int* counts,
                        double* hist);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): fc_hist_accumulate with x, y (length n) and the counts /
+ * hist rows (nbins each, accumulated in place) in caller-owned device
+ * memory.  It calls no hipMalloc, hipFree or allocating routine: its partial
+ * sums live in a fixed 8 MiB device array per GPU, so nbins is limited to
+ * 682 (FC_ERR_INVALID beyond).  Work is queued on ctx's stream and not waited
+ * for (fc_ctx_synchronize).  Results equal fc_hist_accumulate's bit for bit.
+ * All calls on one GPU share the array: calls that overlap in time on the
+ * same GPU (several contexts or host threads) race and must be serialised by
+ * the caller. */
+int fc_hist_accumulate_device_noalloc(fc_ctx* ctx, double minval, double binsize, int nbins,
+                                      const double* x_dev, const double* y_dev, int64_t n,
+                                      int* counts_dev, double* hist_dev);
 
 /* ---- 3D FFT on the device (FftGrid, cc/grid/fft_grid.cc:69-178) ---------- */
 /* Unnormalised R2C of a real [n0,n1,n2] float64 host array into complex128
