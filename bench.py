@@ -80,12 +80,16 @@ def stage_bytes(cfg, s_work, s_grid, kshape, rs, s_inv=None):
     must read and write once.  Complex half grid n^2 (n/2+1) elements of
     2 s_work bytes; real grid n^3 of s_grid; compact k-subgrid values
     ks0 ks1 (ks2/2+1) of s_work; pruned inverse arrays rs n (n/2+1) and
-    rs^2 (n/2+1) complex and the r subgrid rs^3 real, of s_inv bytes."""
+    rs^2 (n/2+1) complex and the r subgrid rs^3 real, of s_inv bytes.  The
+    histograms read, per subgrid cell, its 4-byte cell index and its value
+    (k: one s_work value; r: one s_inv value and three FP64 multipole
+    weights)."""
     s_inv = s_inv or s_work
     n = cfg["n"]
     nh = n // 2 + 1
     half = n * n * nh * 2 * s_work
     pk = kshape[0] * kshape[1] * (kshape[2] // 2 + 1) * s_work
+    K = kshape[0] * kshape[1] * kshape[2]
     q1 = rs * n * nh * 2 * s_inv
     q2 = rs * rs * nh * 2 * s_inv
     xs = rs ** 3 * s_inv
@@ -93,10 +97,10 @@ def stage_bytes(cfg, s_work, s_grid, kshape, rs, s_inv=None):
         "fft_z_fwd": n ** 3 * s_grid + half,
         "fft_y_fwd": 2 * half,
         "combine": 2 * half + pk + q1,  # fused x pass: read a, b; write pk, pruned q1
-        "hist_k": pk,
+        "hist_k": K * (4 + s_work),
         "fft_y_inv": q1 + q2,
         "fft_z_inv": q2 + xs,
-        "hist_r": xs,
+        "hist_r": rs ** 3 * (4 + s_inv + 3 * 8),
     }
 
 
@@ -326,6 +330,14 @@ def serving(lib, corr_mod):
         for f in [srv.submit(j) for j in jobs]:
             f.result()
         burst = len(jobs) / (time.perf_counter() - tb)
+    # The same burst through one slot (one stream): what the concurrent slots add.
+    with CorrelationServer(slots=1, precision="mixed") as srv:
+        for f in [srv.submit(j) for j in warm[::c["slots"]]]:
+            f.result()
+        tb = time.perf_counter()
+        for f in [srv.submit(j) for j in jobs]:
+            f.result()
+        burst1 = len(jobs) / (time.perf_counter() - tb)
     # FP64 plain, one job after another (correlators cached per geometry).
     cache = {}
     tp = 0.0
@@ -353,6 +365,7 @@ def serving(lib, corr_mod):
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "throughput_jobs_per_s": len(jobs) / (max(r.finished for r in res) - t0),
             "burst_capacity_jobs_per_s": burst,
+            "burst_capacity_1_slot_jobs_per_s": burst1,
             "fp64_plain_serial_jobs_per_s": len(jobs) / tp,
             "burst_over_fp64_plain": burst / (len(jobs) / tp)}
 
@@ -373,23 +386,157 @@ def multipole_err(res, ref, key):
     return floor_err(x, r)
 
 
-def cpu_baseline(grids, cfg):
-    """FP64 oracle (scipy.fft with all host threads) on one interlaced
-    correlation of the same grids."""
+def cpu_baseline(grids, cfg, reps=5):
+    """FP64 oracle on interlaced correlations of the same grids: with its FFTs
+    on all host threads (scipy.fft), 1 warm-up then `reps` timed runs,
+    median; and the reference's own threading (its 3D FFTW plan is single
+    threaded, cc/grid/fft_grid.cc:81-84), one timed run on one thread."""
     from oracle import fftcorr_oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     threads = max(1, min(threads, os.cpu_count() or 1))
-    O.FFT_WORKERS = threads
     n = cfg["n"]
     da = grids[0].download()
     db = grids[1].download()
     ref = O.PeriodicCorrelator((n, n, n), cfg["box"] / n, cfg["window"], cfg["rmax"], cfg["dr"],
                                cfg["kmax"], cfg["dk"], cfg["maxell"])
     ref.fast_hist = True
+    old = O.FFT_WORKERS
+    try:
+        O.FFT_WORKERS = threads
+        out = ref.autocorrelate_interlaced(da, db)  # warm-up
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = ref.autocorrelate_interlaced(da, db)
+            times.append(time.perf_counter() - t0)
+        O.FFT_WORKERS = 1
+        t0 = time.perf_counter()
+        ref.autocorrelate_interlaced(da, db)
+        t1 = time.perf_counter() - t0
+    finally:
+        O.FFT_WORKERS = old
+    return float(np.median(times)), threads, out, times, t1
+
+
+C1_CFG = dict(n=128, box=1000.0, npart=100_000, seed=1, rmax=200.0, dr=5.0, kmax=0.03, dk=0.01,
+              reps=5)
+
+
+def config1(lib, corr_mod, ctx, steps):
+    """The reference's CPU-runnable case: 100,000 uniform points (seed 1) in a
+    periodic 1000 Mpc/h box, CIC onto 128^3, xi_0 with r bins 0-200 by 5 and
+    k bins 0-0.03 by 0.01, FP64 -- the GPU FP64 (fused) form against the
+    FP64 oracle on one thread (as the reference runs) and on all threads, 1
+    warm-up then 5 timed runs each, medians."""
+    from fftcorr_amd.catalog import uniform_catalog
+    from oracle import fftcorr_oracle as O
+    c = C1_CFG
+    n = c["n"]
+    cell = c["box"] / n
+    pos = np.ascontiguousarray(uniform_catalog(c["npart"], 0.0, c["box"], c["seed"]))
+    g = lib.DeviceGrid((n, n, n), (0, 0, 0), cell, "fp64", ctx)
+    st = lib.MaStats()
+    lib.check(lib.lib().fc_mass_assign(g.handle, lib.dptr(pos), pos.shape[0], 3, None, 1.0, 3, 1,
+                                       0.0, st))
+    g.normalize(st.totw / float(n) ** 3)
+    args = ((n, n, n), cell, 0, c["rmax"], c["dr"], c["kmax"], c["dk"], 0)
+    corr = corr_mod.PeriodicCorrelator(*args, precision="fp64")
+    L = lib.lib()
+    lib.check(L.fc_corr_autocorrelate(corr._handle, g.handle))
+    ctx.synchronize()
     t0 = time.perf_counter()
-    out = ref.autocorrelate_interlaced(da, db)
-    dt = time.perf_counter() - t0
-    return dt, threads, out
+    for _ in range(steps):
+        lib.check(L.fc_corr_autocorrelate(corr._handle, g.handle))
+    ctx.synchronize()
+    gpu = steps / (time.perf_counter() - t0)
+    corr._fetch()
+    res = corr.results()
+    delta = g.download()
+    ref = O.PeriodicCorrelator(*args)
+    ref.fast_hist = True
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    rates = {}
+    old = O.FFT_WORKERS
+    try:
+        for nt in (1, threads):
+            O.FFT_WORKERS = nt
+            want = ref.autocorrelate(delta)
+            ts = []
+            for _ in range(c["reps"]):
+                t0 = time.perf_counter()
+                want = ref.autocorrelate(delta)
+                ts.append(time.perf_counter() - t0)
+            rates[nt] = 1.0 / float(np.median(ts))
+    finally:
+        O.FFT_WORKERS = old
+    err = max(float(floor_err(res["r_hist"], want["r_hist"]).max()),
+              float(floor_err(res["k_hist"], want["k_hist"]).max()))
+    return {"gpu_fp64": gpu, "unit": "correlations/s",
+            "cpu_port_1_thread": rates[1], "cpu_port_threads": rates[threads],
+            "cpu_threads": threads,
+            "counts_bit_exact": bool(np.array_equal(res["r_counts"], want["r_counts"])),
+            "gpu_vs_cpu_max_floor_rel_err": err,
+            "config": "periodic 128^3 CIC, 100,000 uniform points (seed 1), 1000 Mpc/h, xi_0, "
+                      "r 0-200 by 5, k 0-0.03 by 0.01, FP64"}
+
+
+BATCH512_CFG = dict(n=512, box=2000.0, parents=80_000, children=100, sigma=3.0, members=8,
+                    rmax=200.0, dr=5.0, kmax=0.5, dk=0.005, maxell=4, window=3, scheme=3, reps=3)
+
+
+def batched_512(lib, corr_mod, ctx, pos0):
+    """The north-star form: a batch of 8 periodic 512^3 correlations (l <= 4,
+    kmax 0.5 dk 0.005, rmax 200 dr 5, CIC, sinc^4 window; the headline
+    catalogue and seven more seeded Thomas catalogues of the same kind) in
+    the mixed form, one HIP-graph launch sequence per batch, against the
+    FP64 plain form run member by member on the same grids."""
+    from fftcorr_amd.catalog import thomas_catalog
+    b = BATCH512_CFG
+    n = b["n"]
+    cell = b["box"] / n
+    args = ((n, n, n), cell, b["window"], b["rmax"], b["dr"], b["kmax"], b["dk"], b["maxell"])
+    batch = corr_mod.PeriodicBatch(*args, nbatch=b["members"], precision="mixed", use_graph=True)
+    try:
+        for i in range(b["members"]):
+            pos = pos0 if i == 0 else np.ascontiguousarray(
+                thomas_catalog(b["parents"], b["children"], b["sigma"], 0.0, b["box"], 40 + i))
+            batch.assign(i, pos, scheme=b["scheme"])
+        batch.autocorrelate()  # captures the graph
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(b["reps"]):
+            batch.autocorrelate()
+        ctx.synchronize()
+        rate = b["reps"] * b["members"] / (time.perf_counter() - t0)
+        plain = corr_mod.PeriodicCorrelator(*args, precision="fp64")
+        plain.algorithm = "plain"
+        g = lib.DeviceGrid((n, n, n), (0, 0, 0), cell, "fp64", ctx)
+        err = 0.0
+        t_plain = 0.0
+        nplain = 2
+        for i in range(nplain):
+            g.upload(batch.grid(i).download())
+            lib.check(lib.lib().fc_corr_autocorrelate(plain._handle, g.handle))  # warm-up
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            lib.check(lib.lib().fc_corr_autocorrelate(plain._handle, g.handle))
+            ctx.synchronize()
+            t_plain += time.perf_counter() - t0
+            plain._fetch()
+            ref = plain.results()
+            got = batch.results(i)
+            err = max(err, float(multipole_err(got, ref, "k_hist").max()),
+                      float(multipole_err(got, ref, "r_hist").max()))
+        plain_rate = nplain / t_plain
+        del g, plain
+    finally:
+        batch.close()
+    return {"value": rate, "unit": "correlations/s", "members": b["members"], "grid": n,
+            "fp64_plain_one_by_one": plain_rate, "batched_mixed_over_fp64_plain": rate / plain_rate,
+            "max_floor_rel_err_vs_fp64_plain": err,
+            "config": "batch of 8 periodic 512^3 l<=4 correlations (no interlacing), CIC, "
+                      "sinc^4 window, kmax 0.5 dk 0.005, rmax 200 dr 5, mixed, HIP graph"}
 
 
 def main():
@@ -490,20 +637,31 @@ def main():
             traffic = tj["stages"][dom]["bytes"]
 
     cpu = None
+    c1 = None
     if not args.no_cpu:
-        log("CPU baseline: FP64 oracle on one correlation")
-        dt, threads, ref = cpu_baseline(g64, cfg)
+        log("CPU baseline: FP64 oracle, 1 warm-up + 5 timed correlations, then one on 1 thread")
+        dt, threads, ref, times, t1 = cpu_baseline(g64, cfg)
         e_cpu = max(float(floor_err(ref64["k_hist"], ref["k_hist"]).max()),
                     float(floor_err(ref64["r_hist"], ref["r_hist"]).max()))
         cpu = {"value": 1.0 / dt, "unit": "correlations/s", "cores": threads, "kind": "port",
-               "sample": "one interlaced 512^3 l<=4 correlation of the same grids, FP64 "
-                         "numpy/scipy oracle (scipy.fft workers=%d), %.1f s" % (threads, dt),
+               "sample": "interlaced 512^3 l<=4 correlations of the same grids, FP64 "
+                         "numpy/scipy oracle (scipy.fft workers=%d): 1 warm-up + %d timed, "
+                         "median %.2f s" % (threads, len(times), dt),
+               "times_s": times,
+               "faithful_1_thread": {"value": 1.0 / t1, "unit": "correlations/s", "cores": 1,
+                                     "sample": "one correlation with every FFT on one thread, "
+                                               "as the reference's 3D FFTW plan runs; %.1f s"
+                                               % t1},
                "gpu_fp64_vs_cpu_max_floor_rel_err": e_cpu}
+        log("config 1: 128^3 CIC xi_0 FP64, GPU and CPU")
+        c1 = config1(lib, corr_mod, ctx, 20)
 
     log("Yamamoto correlator at 512^3 (anisotropic stress case)")
     yama_report = yamamoto_512(lib, corr_mod, ctx, pos)
     log("batched mocks: 64 x 256^3 TSC, mixed, HIP graph")
     batch_report = batched_mocks(lib, corr_mod, ctx, 5)
+    log("batched 512^3: 8 members, mixed, HIP graph, against FP64 plain")
+    batch512_report = batched_512(lib, corr_mod, ctx, pos)
     log("serving: Poisson stream of mixed-size jobs")
     serve_report = serving(lib, corr_mod)
 
@@ -560,6 +718,8 @@ def main():
                     "then the correlation; not the headline"},
         "yamamoto_512": yama_report,
         "batched_256": batch_report,
+        "batched_512": batch512_report,
+        "config1_128": c1,
         "serving": serve_report,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": (achieved / PEAK_HBM_GBS) if achieved else None,

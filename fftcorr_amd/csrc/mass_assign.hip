@@ -11,8 +11,10 @@
 //   2. stable radix sort of (key, particle index): within a key cell the
 //      particles keep their input order;
 //   3. k_permute      records in sorted order;
-//   4. k_cell_runs    per key cell, its first sorted particle and its count
-//                     (the count table is cleared first: count 0 = empty);
+//   4. k_cell_runs    per occupied key cell, its first sorted particle and
+//                     its count, and its bit in an occupancy bitmap (cleared
+//                     first; the count table is read only where the bit is
+//                     set, so it needs no clearing);
 //   5. k_ma_tile      one workgroup per 4 x 8 x 32 tile of target cells lists
 //                     the particles of every key cell whose stencil reaches
 //                     the tile, in key-cell order; each wave takes a contiguous
@@ -179,10 +181,11 @@ __global__ void k_zero_i32(int32_t* __restrict__ p, int64_t n) {
 // Runs of equal keys in the sorted keys[0, nvalid): the first particle of
 // each run writes beg[key] = its position and cnt[key] = the run length,
 // found by an exponential then a binary search (a long run costs its first
-// thread O(log length), not O(length)).  Cells without particles keep cnt 0.
+// thread O(log length), not O(length)), and sets the key's occupancy bit.
 __global__ void k_cell_runs(const uint32_t* __restrict__ keys,
                             const unsigned long long* __restrict__ nvalid_p,
-                            int32_t* __restrict__ beg, int32_t* __restrict__ cnt) {
+                            int32_t* __restrict__ beg, int32_t* __restrict__ cnt,
+                            uint32_t* __restrict__ occ) {
   const int64_t nvalid = (int64_t)*nvalid_p;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvalid;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -203,7 +206,14 @@ __global__ void k_cell_runs(const uint32_t* __restrict__ keys,
     }
     beg[k] = (int32_t)i;
     cnt[k] = (int32_t)(lo + 1 - i);
+    atomicOr(&occ[k >> 5], 1u << (k & 31));
   }
+}
+
+// Particles in key cell c: 0 unless its occupancy bit is set.
+__device__ __forceinline__ int cell_count(const uint32_t* __restrict__ occ,
+                                          const int32_t* __restrict__ cnt, int64_t c) {
+  return ((occ[c >> 5] >> (c & 31)) & 1u) ? cnt[c] : 0;
 }
 
 // v mod n for the halo of a tile: one compare in the common case, the full
@@ -250,8 +260,8 @@ struct Stencil<3> {  // CIC about cell centres: key cell a <= x - 1/2 < a + 1
 template <typename Tg, int SCHEME>
 __global__ void __launch_bounds__(kTileThreads)
     k_ma_tile(Tg* __restrict__ grid, GridGeom g, const int32_t* __restrict__ beg,
-              const int32_t* __restrict__ cnt, const double4* __restrict__ srec, int tiles_y,
-              int tiles_z) {
+              const int32_t* __restrict__ cnt, const uint32_t* __restrict__ occ,
+              const double4* __restrict__ srec, int tiles_y, int tiles_z) {
 #pragma clang fp contract(off)
   using S = Stencil<SCHEME>;
   constexpr int W = S::OMAX - S::OMIN + 1;
@@ -279,7 +289,7 @@ __global__ void __launch_bounds__(kTileThreads)
     const int gy = wrap_index(y0 - S::OMAX + ry, g.n1);
     const int gz = wrap_index(z0 - S::OMAX + rz, g.n2);
     const int64_t c = ((int64_t)gx * g.n1 + gy) * g.n2 + gz;
-    const int m = cnt[c];
+    const int m = cell_count(occ, cnt, c);
     pre[rc] = m;
     if (m > 0) bg[rc] = beg[c];
   }
@@ -412,7 +422,7 @@ constexpr int DX = WTX + WC - 1, DY = WTY + WC - 1, DZ = WTZ + WC - 1;
 template <typename Tg>
 __global__ void __launch_bounds__(WTX* WTY* WTZ)
     k_ma_gather_wavelet(Tg* __restrict__ grid, GridGeom g, const int32_t* __restrict__ beg,
-                        const int32_t* __restrict__ cnt,
+                        const int32_t* __restrict__ cnt, const uint32_t* __restrict__ occ,
                         const double4* __restrict__ srec, const double* __restrict__ table,
                         int tiles_y, int tiles_z) {
 #pragma clang fp contract(off)
@@ -432,7 +442,7 @@ __global__ void __launch_bounds__(WTX* WTY* WTZ)
     const int gy = wrap_index(y0 + hy - (WC - 2), g.n1);
     const int gz = wrap_index(z0 + hz - (WC - 2), g.n2);
     const int64_t c = ((int64_t)gx * g.n1 + gy) * g.n2 + gz;
-    const int m = cnt[c];
+    const int m = cell_count(occ, cnt, c);
     cs[h] = m > 0 ? beg[c] : 0;
     ce[h] = cs[h] + m;
   }
@@ -485,7 +495,7 @@ const double* wavelet_on_device(fc_ctx* ctx) {
 
 template <typename Tg>
 void launch_gather(int scheme, Tg* grid, const GridGeom& gg, const int32_t* beg,
-                   const int32_t* cnt, const double4* srec, hipStream_t s,
+                   const int32_t* cnt, const uint32_t* occ, const double4* srec, hipStream_t s,
                    const double* table = nullptr) {
   const bool wav = scheme == 2;
   const int bx = wav ? WTX : TX, by = wav ? WTY : TY, bz = wav ? WTZ : TZ;
@@ -495,17 +505,20 @@ void launch_gather(int scheme, Tg* grid, const GridGeom& gg, const int32_t* beg,
   dim3 grd((unsigned)nb), blk(wav ? WTX * WTY * WTZ : kTileThreads);
   switch (scheme) {
     case 0:
-      hipLaunchKernelGGL((k_ma_tile<Tg, 0>), grd, blk, 0, s, grid, gg, beg, cnt, srec, ty, tz);
+      hipLaunchKernelGGL((k_ma_tile<Tg, 0>), grd, blk, 0, s, grid, gg, beg, cnt, occ, srec, ty,
+                         tz);
       break;
     case 1:
-      hipLaunchKernelGGL((k_ma_tile<Tg, 1>), grd, blk, 0, s, grid, gg, beg, cnt, srec, ty, tz);
+      hipLaunchKernelGGL((k_ma_tile<Tg, 1>), grd, blk, 0, s, grid, gg, beg, cnt, occ, srec, ty,
+                         tz);
       break;
     case 3:
-      hipLaunchKernelGGL((k_ma_tile<Tg, 3>), grd, blk, 0, s, grid, gg, beg, cnt, srec, ty, tz);
+      hipLaunchKernelGGL((k_ma_tile<Tg, 3>), grd, blk, 0, s, grid, gg, beg, cnt, occ, srec, ty,
+                         tz);
       break;
     case 2:
-      hipLaunchKernelGGL((k_ma_gather_wavelet<Tg>), grd, blk, 0, s, grid, gg, beg, cnt, srec,
-                         table, ty, tz);
+      hipLaunchKernelGGL((k_ma_gather_wavelet<Tg>), grd, blk, 0, s, grid, gg, beg, cnt, occ,
+                         srec, table, ty, tz);
       break;
   }
   FC_LAUNCH_CHECK();
@@ -561,10 +574,13 @@ void mass_assign(fc_grid* grid, const double* pos_dev, int64_t n, int ncol,
   sc[6].reserve(sizeof(double4) * n);
   sc[7].reserve(sizeof(double) * 2 * (nbp + 1));
   sc[8].reserve(sizeof(unsigned long long) * (nbp + 1));
+  const uint64_t nwords = (ncells + 31) / 32;
   sc[9].reserve(sizeof(int32_t) * ncells);
   sc[10].reserve(sizeof(int32_t) * ncells);
+  sc[11].reserve(sizeof(uint32_t) * nwords);
   DeviceBuffer &keys = sc[1], &keys2 = sc[2], &idx = sc[3], &idx2 = sc[4], &rec = sc[5],
-               &srec = sc[6], &partw = sc[7], &partn = sc[8], &beg = sc[9], &cnt = sc[10];
+               &srec = sc[6], &partw = sc[7], &partn = sc[8], &beg = sc[9], &cnt = sc[10],
+               &occ = sc[11];
   const double anchor = scheme == 3 ? 0.5 : 0.0;  // CIC keys by its 2x2x2 block
 
   hipLaunchKernelGGL(k_ma_prepare, dim3(nbp), dim3(kPrepThreads), 0, s, pos_dev, n, ncol, w_dev,
@@ -591,20 +607,20 @@ void mass_assign(fc_grid* grid, const double* pos_dev, int64_t n, int ncol,
   hipLaunchKernelGGL(k_permute, dim3(grid1d(n, 256, 65536)), dim3(256), 0, s,
                      idx2.as<uint32_t>(), rec.as<double4>(), srec.as<double4>(), n);
   FC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_zero_i32, dim3(grid1d(ncells, 256, 65536)), dim3(256), 0, s,
-                     cnt.as<int32_t>(), (int64_t)ncells);
+  hipLaunchKernelGGL(k_zero_i32, dim3(grid1d(nwords, 256, 65536)), dim3(256), 0, s,
+                     occ.as<int32_t>(), (int64_t)nwords);
   FC_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_cell_runs, dim3(grid1d(n, 256, 65536)), dim3(256), 0, s,
                      keys2.as<uint32_t>(), partn.as<unsigned long long>() + nbp,
-                     beg.as<int32_t>(), cnt.as<int32_t>());
+                     beg.as<int32_t>(), cnt.as<int32_t>(), occ.as<uint32_t>());
   FC_LAUNCH_CHECK();
   FC_HIP(hipEventRecord(ev[1], s));
   if (grid->prec == FC_FP64)
     launch_gather<double>(scheme, grid->data.as<double>(), gg, beg.as<int32_t>(),
-                          cnt.as<int32_t>(), srec.as<double4>(), s, wtab);
+                          cnt.as<int32_t>(), occ.as<uint32_t>(), srec.as<double4>(), s, wtab);
   else
     launch_gather<float>(scheme, grid->data.as<float>(), gg, beg.as<int32_t>(),
-                         cnt.as<int32_t>(), srec.as<double4>(), s, wtab);
+                         cnt.as<int32_t>(), occ.as<uint32_t>(), srec.as<double4>(), s, wtab);
   FC_HIP(hipEventRecord(ev[2], s));
 
   double wsum[2];
