@@ -56,6 +56,35 @@ template <typename To, typename C>
 __device__ __forceinline__ cplx<To> to_cplx(C a) {
   return cplx<To>{To(a.x), To(a.y)};
 }
+// Streaming global accesses for the whole-grid passes: every element is read
+// once and written once per pass and the grids are far larger than the
+// caches, so these loads and stores carry the nontemporal hint.
+template <typename V>
+struct NtVec;
+template <>
+struct NtVec<float2> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct NtVec<double2> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <typename V>
+__device__ __forceinline__ V ld_stream(const V* p) {
+  const auto e = __builtin_nontemporal_load(reinterpret_cast<const typename NtVec<V>::type*>(p));
+  V v;
+  v.x = e.x;
+  v.y = e.y;
+  return v;
+}
+template <typename V>
+__device__ __forceinline__ void st_stream(V* p, V v) {
+  typename NtVec<V>::type e;
+  e.x = v.x;
+  e.y = v.y;
+  __builtin_nontemporal_store(e, reinterpret_cast<typename NtVec<V>::type*>(p));
+}
+
 template <typename C>
 struct ScalarOf;
 template <>
@@ -485,8 +514,8 @@ __attribute__((amdgpu_waves_per_eu(rc_wide_waves<T, To>())))
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     if (colok) {
-      const cplx<Tin> x =
-          *reinterpret_cast<const cplx<Tin>*>(srcb + (cin + (uint32_t)(t + s * TCOL) * inLb));
+      const cplx<Tin> x = ld_stream(
+          reinterpret_cast<const cplx<Tin>*>(srcb + (cin + (uint32_t)(t + s * TCOL) * inLb)));
       v[0][s] = C{T(x.x), T(x.y)};
     } else {
       v[0][s] = C{T(0), T(0)};
@@ -499,7 +528,8 @@ __attribute__((amdgpu_waves_per_eu(rc_wide_waves<T, To>())))
     int o = t + out_m<L>(s) * TCOL + offm;
     if (o >= L) o -= L;
     for (; o < nout; o += L)
-      *reinterpret_cast<cplx<To>*>(dstb + (cout + (uint32_t)o * outLb)) = to_cplx<To>(v[0][s]);
+      st_stream(reinterpret_cast<cplx<To>*>(dstb + (cout + (uint32_t)o * outLb)),
+                to_cplx<To>(v[0][s]));
   }
 }
 
@@ -556,7 +586,7 @@ __global__ void __launch_bounds__(1024)
       const int e = t + s * TCOL;
       C z{T(0), T(0)};
       if (rowok) {
-        const P p = src[e];
+        const P p = ld_stream(src + e);
         z.x = T(ld((double)p.x, row, 2 * e));
         z.y = T(ld((double)p.y, row, 2 * e + 1));
       }
@@ -583,7 +613,7 @@ __global__ void __launch_bounds__(1024)
     if constexpr (sizeof(T) != 8) w = twN[k];
     const T px = w.x * by + w.y * bx;
     const T py = -(w.x * bx - w.y * by);
-    dst[k] = to_cplx<To>(C{T(0.5) * (ax + px), T(0.5) * (ay + py)});
+    st_stream(dst + k, to_cplx<To>(C{T(0.5) * (ax + px), T(0.5) * (ay + py)}));
     if constexpr (sizeof(T) == 8) w = cmul(w, wstep);
   }
 }

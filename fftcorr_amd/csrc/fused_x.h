@@ -137,8 +137,9 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     const uint32_t o = base + (uint32_t)(t + s * TCOL) * sL;
-    v[0][s] = colok ? *reinterpret_cast<const C*>(Ab + o) : C{T(0), T(0)};
-    if (NT == 2) v[NT - 1][s] = colok ? *reinterpret_cast<const C*>(Bb + o) : C{T(0), T(0)};
+    v[0][s] = colok ? ld_stream(reinterpret_cast<const C*>(Ab + o)) : C{T(0), T(0)};
+    if (NT == 2)
+      v[NT - 1][s] = colok ? ld_stream(reinterpret_cast<const C*>(Bb + o)) : C{T(0), T(0)};
   }
   if (MODE == kModeCross) {
 #pragma unroll
@@ -206,9 +207,11 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
       int sx = t + out_m<L>(s) * TCOL + offm;
       if (sx >= L) sx -= L;
       for (; sx < rs; sx += L)
-        *reinterpret_cast<CI*>(reinterpret_cast<char*>(q) +
-                               (((uint32_t)sx * (uint32_t)n1 + (uint32_t)y) * (uint32_t)ldq +
-                                (uint32_t)cg) * (uint32_t)sizeof(CI)) = u[0][s];
+        st_stream(reinterpret_cast<CI*>(reinterpret_cast<char*>(q) +
+                                        (((uint32_t)sx * (uint32_t)n1 + (uint32_t)y) *
+                                             (uint32_t)ldq +
+                                         (uint32_t)cg) * (uint32_t)sizeof(CI)),
+                  u[0][s]);
     }
   }
 }
