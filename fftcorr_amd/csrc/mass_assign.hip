@@ -283,15 +283,41 @@ __global__ void __launch_bounds__(kTileThreads)
   const int x0 = bx * TX, y0 = by * TY, z0 = bz * TZ;
   const int tid = threadIdx.x, lane = tid % kWave, wv = tid / kWave;
 
-  for (int rc = tid; rc < RC; rc += kTileThreads) {
-    const int rz = rc % RZ, ry = (rc / RZ) % RY, rx = rc / (RZ * RY);
-    const int gx = wrap_index(x0 - S::OMAX + rx, g.n0);
-    const int gy = wrap_index(y0 - S::OMAX + ry, g.n1);
-    const int gz = wrap_index(z0 - S::OMAX + rz, g.n2);
-    const int64_t c = ((int64_t)gx * g.n1 + gy) * g.n2 + gz;
-    const int m = cell_count(occ, cnt, c);
-    pre[rc] = m;
-    if (m > 0) bg[rc] = beg[c];
+  // Run tables of the key cells: every occupancy word is requested before the
+  // first is used, then the runs of the occupied cells, so a thread waits for
+  // two rounds of loads rather than one per cell.
+  {
+    int64_t cell[kPer];
+    uint32_t word[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int rc = tid + q * kTileThreads;
+      cell[q] = -1;
+      word[q] = 0u;
+      if (rc < RC) {
+        const int rz = rc % RZ, ry = (rc / RZ) % RY, rx = rc / (RZ * RY);
+        const int gx = wrap_index(x0 - S::OMAX + rx, g.n0);
+        const int gy = wrap_index(y0 - S::OMAX + ry, g.n1);
+        const int gz = wrap_index(z0 - S::OMAX + rz, g.n2);
+        cell[q] = ((int64_t)gx * g.n1 + gy) * g.n2 + gz;
+        word[q] = occ[cell[q] >> 5];
+      }
+    }
+    int m[kPer], b0[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const bool on = cell[q] >= 0 && ((word[q] >> (cell[q] & 31)) & 1u);
+      m[q] = on ? cnt[cell[q]] : 0;
+      b0[q] = on ? beg[cell[q]] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int rc = tid + q * kTileThreads;
+      if (rc < RC) {
+        pre[rc] = m[q];
+        bg[rc] = b0[q];
+      }
+    }
   }
   __syncthreads();
   // Exclusive scan of the counts: thread t owns entries [t kPer, (t+1) kPer).
@@ -399,14 +425,26 @@ __global__ void __launch_bounds__(kTileThreads)
     }
   }
   __syncthreads();
-  for (int t = tid; t < kTileCells; t += kTileThreads) {
-    if (!((touched[t >> 5] >> (t & 31)) & 1u)) continue;
+  // Each thread's touched cells: all grid loads first, then the stores.
+  constexpr int kOut = kTileCells / kTileThreads;
+  Tg old[kOut];
+  Tg* dst[kOut];
+#pragma unroll
+  for (int q = 0; q < kOut; ++q) {
+    const int t = tid + q * kTileThreads;
+    const int tz = t % TZ, ty = (t / TZ) % TY, tx = t / (TZ * TY);
+    const bool on = (touched[t >> 5] >> (t & 31)) & 1u;
+    dst[q] = on ? grid + ((int64_t)(x0 + tx) * g.n1 + (y0 + ty)) * g.n2 + (z0 + tz) : nullptr;
+    old[q] = on ? *dst[q] : Tg(0);
+  }
+#pragma unroll
+  for (int q = 0; q < kOut; ++q) {
+    if (!dst[q]) continue;
+    const int t = tid + q * kTileThreads;
     double sum = acc[0][t];
 #pragma unroll
     for (int w = 1; w < kTileWaves; ++w) sum += acc[w][t];
-    const int tz = t % TZ, ty = (t / TZ) % TY, tx = t / (TZ * TY);
-    Tg* cell = grid + ((int64_t)(x0 + tx) * g.n1 + (y0 + ty)) * g.n2 + (z0 + tz);
-    *cell = (Tg)((double)*cell + sum);
+    *dst[q] = (Tg)((double)old[q] + sum);
   }
 }
 
@@ -665,6 +703,45 @@ int fc_set_wavelet_table(const double* table, int n) {
     else
       fc::g_wavelet.clear();
     ++fc::g_wavelet_ver;
+  });
+}
+
+int fc_mass_assign_footprint(fc_grid* grid, int64_t n, int ncol, int has_weights,
+                             int host_input, size_t* grow, size_t* transient) {
+  return fc::guard([&] {
+    fc::check_grid(grid);
+    if (n < 0) fc::fail(FC_ERR_INVALID, "negative particle count");
+    if (ncol != 3 && ncol != 4) fc::fail(FC_ERR_INVALID, "Rows must be 3D or 4D.");
+    fc_ctx* ctx = grid->ctx;
+    fc::DeviceGuard dg(ctx->device);
+    size_t g = 0;
+    if (n > 0) {
+      const uint64_t ncells = grid->size();
+      const int nbp = (int)std::min<int64_t>(1024, (n + fc::kPrepThreads - 1) / fc::kPrepThreads);
+      size_t temp = 0;
+      FC_HIP(hipcub::DeviceRadixSort::SortPairs(
+          nullptr, temp, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+          (uint32_t*)nullptr, (int)n, 0, fc::key_bits(ncells), ctx->stream));
+      const size_t want[12] = {temp + 256,
+                               sizeof(uint32_t) * n,
+                               sizeof(uint32_t) * n,
+                               sizeof(uint32_t) * n,
+                               sizeof(uint32_t) * n,
+                               sizeof(double4) * n,
+                               sizeof(double4) * n,
+                               sizeof(double) * 2 * (nbp + 1),
+                               sizeof(unsigned long long) * (nbp + 1),
+                               sizeof(int32_t) * ncells,
+                               sizeof(int32_t) * ncells,
+                               sizeof(uint32_t) * ((ncells + 31) / 32)};
+      for (int i = 0; i < 12; ++i)
+        if (want[i] > ctx->scratch[i].bytes()) g += want[i] - ctx->scratch[i].bytes();
+    }
+    if (grow) *grow = g;
+    if (transient)
+      *transient = host_input && n > 0
+                       ? sizeof(double) * (size_t)n * (ncol + (has_weights ? 1 : 0))
+                       : 0;
   });
 }
 
