@@ -61,8 +61,18 @@ SIGNATURES = {
     "fc_device_count": (ctypes.c_int, [_ip]),
     "fc_device_memory": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_size_t),
                                         ctypes.POINTER(ctypes.c_size_t)]),
+    "fc_device_bytes": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_size_t),
+                                       ctypes.POINTER(ctypes.c_size_t)]),
+    "fc_device_bytes_reset_peak": (ctypes.c_int, [ctypes.c_int]),
+    "fc_mass_assign_footprint": (ctypes.c_int, [_vp, _i64, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, ctypes.POINTER(ctypes.c_size_t),
+                                                ctypes.POINTER(ctypes.c_size_t)]),
+    "fc_corr_footprint": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t),
+                                         ctypes.POINTER(ctypes.c_size_t)]),
     "fc_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     "fc_ctx_destroy": (ctypes.c_int, [_vp]),
+    "fc_ctx_footprint": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    "fc_ctx_release_scratch": (ctypes.c_int, [_vp]),
     "fc_ctx_synchronize": (ctypes.c_int, [_vp]),
     "fc_grid_create": (ctypes.c_int, [_vp, _ip, _dp, ctypes.c_double, ctypes.c_int,
                                       ctypes.POINTER(_vp)]),
@@ -238,6 +248,19 @@ def context(device=None):
         with _lock:
             c = _contexts.setdefault(device, c)
     return c
+
+
+def device_bytes(device=None):
+    """(bytes the library holds on the device now, peak since the last reset)."""
+    device = default_device() if device is None else int(device)
+    a, b = ctypes.c_size_t(), ctypes.c_size_t()
+    check(lib().fc_device_bytes(device, ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
+
+
+def reset_device_peak(device=None):
+    device = default_device() if device is None else int(device)
+    check(lib().fc_device_bytes_reset_peak(device))
 
 
 def device_count():

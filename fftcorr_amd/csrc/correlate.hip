@@ -277,6 +277,37 @@ struct KEntry {
   }
 };
 
+// Yamamoto k space, one (l, m) transform: the value is Re P at the cell's
+// compact slot (the reflected slot for kz < 0) and the weight
+// Y_lm(k) coef W^-1(k), evaluated on the fly with the operations of the
+// subgrid accumulation.  The histograms of the (l, m) transforms of one l add
+// up to the histogram of the k subgrid summed over m (correlator.h:453-466),
+// without the FP64 k subgrid in memory.
+template <typename T>
+struct KEntryLm {
+  const uint32_t* cells;
+  const T* pk;
+  KSlots ks;
+  FastDiv d2, d1;  // division by ks2 and ks1
+  const double *kx, *ky, *kz, *wx, *wy, *wz;
+  int ell, m;
+  double coef;
+  __device__ __forceinline__ void offset(int) {}
+  __device__ __forceinline__ void operator()(int64_t pos, double* v) const {
+#pragma clang fp contract(off)
+    const uint32_t cell = cells[pos];
+    const uint32_t t = d2.div(cell);
+    const int k = (int)(cell - t * (uint32_t)ks.ks2);
+    const uint32_t iu = d1.div(t);
+    const int j = (int)(t - iu * (uint32_t)ks.ks1), i = (int)iu;
+    const int oz = ks.oz();
+    const int64_t src = k >= oz ? ks.at(i, j, k - oz) : ks.at(ks.ks0 - 1 - i, ks.ks1 - 1 - j, oz - k);
+    const double x = (double)pk[src];
+    const double invw = 1.0 / (wx[i] * wy[j] * wz[k]);
+    v[0] = ylm_eval(ell, m, kx[i], ky[j], kz[k], coef, invw) * x;
+  }
+};
+
 // Precomputed weights wt[q][pos] (periodic r space).
 template <typename V, int NV>
 struct WEntry {
@@ -992,10 +1023,10 @@ void hist_subgrids(fc_corr* c, int ell, bool k_side, bool r_side) {
   }
 }
 
-void ensure_subgrids(fc_corr* c) {
+void ensure_subgrids(fc_corr* c, bool ksub = true) {
   const int rs = c->rshape;
   c->rsub.reserve(sizeof(double) * (size_t)rs * rs * rs);
-  c->ksub.reserve(sizeof(double) * (size_t)c->kshape[0] * c->kshape[1] * c->kshape[2]);
+  if (ksub) c->ksub.reserve(sizeof(double) * (size_t)c->kshape[0] * c->kshape[1] * c->kshape[2]);
 }
 
 // PeriodicCorrelator::correlate_internal (correlator.h:273-357) stage by stage.
@@ -1064,7 +1095,7 @@ void yamamoto_correlate(fc_corr* c, fc_grid* g1) {
   const uint64_t ncells = (uint64_t)c->shape[0] * c->shape[1] * c->shape[2];
   ensure_periodic_buffers<T, TI>(c);
   ensure_phase_tables(c);
-  ensure_subgrids(c);
+  ensure_subgrids(c, false);
   // Observer at the survey origin; cell centres 0.5 - observer + i
   // (correlator.h:381-402).
   double start[3];
@@ -1076,7 +1107,19 @@ void yamamoto_correlate(fc_corr* c, fc_grid* g1) {
       LoadYlm ld{ell, m, p.n[1], ylm_coeff(ell, m, 1.0), start[0], start[1], start[2]};
       forward_zy<T, TI>(c, g1, w, ld);
       power_and_pruned_inverse<T, TI>(c, w, c->grid2_fft.as<cplx<T>>(), kModeCross);
-      plain_k_multipole<T>(c, ell, m, 4.0 * M_PI, m != -ell);
+      {
+        // k-space multipole l: this transform's share, straight from the
+        // compact k values (counts once per l).
+        Stage st(c, FC_STAGE_HIST_K);
+        KEntryLm<T> ke{c->kbin.cells.as<uint32_t>(), c->pk.as<T>(), kslots(c),
+                       FastDiv((uint32_t)c->kshape[2]), FastDiv((uint32_t)c->kshape[1]),
+                       c->kx.as<double>(), c->ky.as<double>(), c->kz.as<double>(),
+                       c->wx.as<double>(), c->wy.as<double>(), c->wz.as<double>(), ell, m,
+                       ylm_coeff(ell, m, 4.0 * M_PI)};
+        const int row = ell / 2;
+        run_hist_seg<1>(ke, c->kbin, c->hs.part_seg, c->k_hist.as<double>(),
+                        c->k_counts.as<int>(), row, row, m == -ell ? row + 1 : row, s);
+      }
       Stage st(c, FC_STAGE_SUBGRID);
       if (m == -ell) fill<double>(c->rsub.as<double>(), nr, 0.0, s);
       hipLaunchKernelGGL(k_acc_r<TI>, dim3(grid1d(nr, 256, 65536)), dim3(256), 0, s,
@@ -1084,7 +1127,7 @@ void yamamoto_correlate(fc_corr* c, fc_grid* g1) {
                          m, ylm_coeff(ell, m, 4.0 * M_PI / ncells));
       FC_LAUNCH_CHECK();
     }
-    hist_subgrids(c, ell, true, true);
+    hist_subgrids(c, ell, false, true);
   }
 }
 
@@ -1207,6 +1250,52 @@ int fc_corr_destroy(fc_corr* corr) {
     if (corr->ev0) (void)hipEventDestroy(corr->ev0);
     if (corr->ev1) (void)hipEventDestroy(corr->ev1);
     delete corr;
+  });
+}
+
+int fc_corr_footprint(fc_corr* c, int interlaced, size_t* held, size_t* full) {
+  return fc::guard([&] {
+    if (!c) fc::fail(FC_ERR_INVALID, "null correlator");
+    // (bytes held now, bytes the buffer reaches once the correlation paths
+    // have run) for every device buffer of the correlator.
+    size_t now = 0, need = 0;
+    auto add = [&](const fc::DeviceBuffer& b, size_t want) {
+      now += b.bytes();
+      need += std::max(b.bytes(), want);
+    };
+    for (const fc::DeviceBuffer* b : {&c->rx, &c->kx, &c->ky, &c->kz, &c->wx, &c->wy, &c->wz,
+                                      &c->r_counts, &c->r_hist, &c->k_counts, &c->k_hist,
+                                      &c->zerolag, &c->hs.part_hist, &c->hs.part_counts})
+      add(*b, 0);
+    for (const fc::BinnedCells* bc : {&c->kbin, &c->rbin}) {
+      add(bc->cells, 0);
+      add(bc->counts, 0);
+      add(bc->chunks, 0);
+      add(bc->chunk_first, 0);
+    }
+    const fc::FftPlan& p = *c->plan;
+    const fc::FftPlan& pi = *c->plan_inv;
+    const size_t cb = p.complex_bytes();
+    const size_t ts = c->prec == FC_FP64 ? 8 : 4;            // forward storage
+    const size_t tsi = c->prec == FC_FP32 ? 4 : 8;           // inverse arithmetic
+    const size_t rs = (size_t)c->rshape;
+    const size_t nv = (size_t)c->nell;
+    const size_t K = (size_t)c->kshape[0] * c->kshape[1] * c->kshape[2];
+    add(c->work, cb);
+    add(c->work_b, interlaced ? cb : 0);
+    add(c->grid2_fft, c->kind == FC_YAMAMOTO || c->have_grid2 ? cb : 0);
+    add(c->pk, ts * (size_t)c->kshape[0] * c->kshape[1] * (c->kshape[2] / 2 + 1));
+    add(c->q1, 2 * tsi * rs * pi.n[1] * pi.ldc);
+    add(c->q2, 2 * tsi * rs * rs * pi.ldc);
+    add(c->xisub, tsi * rs * rs * rs);
+    add(c->rsub, c->kind == FC_YAMAMOTO || c->algo == FC_ALGO_PLAIN ? 8 * rs * rs * rs : 0);
+    add(c->ksub, c->algo == FC_ALGO_PLAIN ? 8 * K : 0);
+    add(c->rw, 8 * nv * (size_t)std::max<int64_t>(c->rbin.n, 1));
+    add(c->hs.part_seg, 8 * nv * (size_t)std::max(c->kbin.nchunks, c->rbin.nchunks));
+    for (int d = 0; d < 3; ++d)
+      add(c->ph[d], sizeof(double2) * (size_t)(d == 2 ? p.ldc : c->shape[d]));
+    if (held) *held = now;
+    if (full) *full = need;
   });
 }
 
@@ -1425,9 +1514,33 @@ void batch_sequence(fc_batch* bt, int n) {
     launch_cols_wide<T, -1, TI>(p.n[1], v, p.tile[1], pi.tw1.as<cplx<TI>>(),
                                 p.tw1.as<cplx<T>>(), s);
   }
-  PxOut po{bt->pk.as<void>(), bt->q1.as<void>(), n, (int64_t)p.n[0] * p.n[1] * p.ldc, bt->npk,
-           bt->nq1};
-  launch_periodic_x2<T, TI, kModeAuto>(c, w, nullptr, po);
+  if (px_fits(c)) {
+    PxOut po{bt->pk.as<void>(), bt->q1.as<void>(), n, (int64_t)p.n[0] * p.n[1] * p.ldc, bt->npk,
+             bt->nq1};
+    launch_periodic_x2<T, TI, kModeAuto>(c, w, nullptr, po);
+  } else {
+    // x lengths without the fused kernel (e.g. 384 = 3 x 128): per member,
+    // the x pass, |F|^2, the compact k values and the pruned inverse x pass
+    // (the single correlator's separate-pass sequence) into its slices.
+    const int64_t nc = p.nrows() * p.ldc;
+    const KSlots ks = kslots(c);
+    const int64_t npk = bt->npk;
+    for (int i = 0; i < n; ++i) {
+      cplx<T>* wi = w + (int64_t)i * nc;
+      launch_cols_wide<T, -1, TI>(p.n[0], x_view<T>(p, wi), p.tile[0], pi.tw0.as<cplx<TI>>(),
+                                  p.tw0.as<cplx<T>>(), s);
+      hipLaunchKernelGGL(k_conj_mult<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, wi,
+                         nullptr, nc);
+      FC_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gather_pk<T>, dim3(grid1d(npk, 256, 65536)), dim3(256), 0, s,
+                         bt->pk.as<T>() + i * npk, wi, ks, p.n[0], p.n[1], (int64_t)p.ldc);
+      FC_LAUNCH_CHECK();
+      ColView<TI, T> v{wi, 0, (int64_t)p.n[1] * p.ldc, (int64_t)p.ldc,
+                       bt->q1.as<cplx<TI>>() + i * bt->nq1, 0, (int64_t)p.n[1] * ldq, ldq, 1,
+                       p.n[1], p.n2h, rs, o};
+      launch_cols<TI, +1, T>(p.n[0], v, pi.tile[0], pi.tw0.as<cplx<TI>>(), s);
+    }
+  }
   {
     ColView<TI> v{bt->q1.as<cplx<TI>>(), (int64_t)pi.n[1] * ldq, ldq, 0, bt->q2.as<cplx<TI>>(),
                   (int64_t)rs * ldq, ldq, 0, n * rs, 1, pi.n2h, rs, o};
@@ -1494,9 +1607,6 @@ int fc_batch_create(fc_ctx* ctx, const int shape[3], double cell_size, int windo
       fc::fail(rc, fc_last_error());
     fc_corr* c = b->corr;
     fc::DeviceGuard dg(ctx->device);
-    if (!fc::px_fits(c))
-      fc::fail(FC_ERR_INVALID,
-               "batches need n0 a power of two from 8 to 512 and members under 4 GB");
     const fc::FftPlan& p = *c->plan;
     const fc::FftPlan& pi = *c->plan_inv;
     const size_t csi = pi.csize();

@@ -45,8 +45,13 @@ void noalloc_scratch_init(int device);
 #define FC_LAUNCH_CHECK() FC_HIP(hipGetLastError())
 
 // Wraps a C-ABI body: converts exceptions into status codes + last error.
+// The HIP runtime's per-thread error state is cleared first, so that the
+// launch checks of this call report only this call's errors (and not one
+// left by an earlier call whose status was not checked, e.g. in a release
+// path).
 template <typename F>
 int guard(F&& f) {
+  (void)hipGetLastError();
   try {
     f();
     return FC_OK;
@@ -62,6 +67,12 @@ int guard(F&& f) {
   }
 }
 
+// Bytes of device memory the library holds per device (every DeviceBuffer
+// allocation), and the peak since the last reset: the measured footprint the
+// serving layer admits against.
+constexpr int kMaxTrackedDevices = 64;
+void note_device_bytes(int device, int64_t delta);
+
 // Owning device allocation (hipMalloc / hipFree).
 class DeviceBuffer {
  public:
@@ -70,7 +81,8 @@ class DeviceBuffer {
   ~DeviceBuffer() { release(); }
   DeviceBuffer(const DeviceBuffer&) = delete;
   DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_), owned_(o.owned_) {
+  DeviceBuffer(DeviceBuffer&& o) noexcept
+      : ptr_(o.ptr_), bytes_(o.bytes_), owned_(o.owned_), dev_(o.dev_) {
     o.ptr_ = nullptr;
     o.bytes_ = 0;
     o.owned_ = true;
@@ -81,6 +93,7 @@ class DeviceBuffer {
       ptr_ = o.ptr_;
       bytes_ = o.bytes_;
       owned_ = o.owned_;
+      dev_ = o.dev_;
       o.ptr_ = nullptr;
       o.bytes_ = 0;
       o.owned_ = true;
@@ -90,15 +103,20 @@ class DeviceBuffer {
   void allocate(size_t bytes) {
     release();
     if (bytes == 0) return;
+    FC_HIP(hipGetDevice(&dev_));
     FC_HIP(hipMalloc(&ptr_, bytes));
     bytes_ = bytes;
+    note_device_bytes(dev_, (int64_t)bytes);
   }
   // Grows (never shrinks); contents are not preserved.
   void reserve(size_t bytes) {
     if (bytes > bytes_) allocate(bytes);
   }
   void release() {
-    if (ptr_ && owned_) (void)hipFree(ptr_);
+    if (ptr_ && owned_) {
+      (void)hipFree(ptr_);
+      note_device_bytes(dev_, -(int64_t)bytes_);
+    }
     ptr_ = nullptr;
     bytes_ = 0;
     owned_ = true;
@@ -120,6 +138,7 @@ class DeviceBuffer {
   void* ptr_ = nullptr;
   size_t bytes_ = 0;
   bool owned_ = true;
+  int dev_ = 0;
 };
 
 }  // namespace fc

@@ -137,9 +137,10 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     const uint32_t o = base + (uint32_t)(t + s * TCOL) * sL;
-    v[0][s] = colok ? ld_stream(reinterpret_cast<const C*>(Ab + o)) : C{T(0), T(0)};
-    if (NT == 2)
-      v[NT - 1][s] = colok ? ld_stream(reinterpret_cast<const C*>(Bb + o)) : C{T(0), T(0)};
+    // Plain loads: the two tiles of a 128-byte line (64 bytes each) are read
+    // by neighbouring workgroups of one XCD, the second from its L2.
+    v[0][s] = colok ? *reinterpret_cast<const C*>(Ab + o) : C{T(0), T(0)};
+    if (NT == 2) v[NT - 1][s] = colok ? *reinterpret_cast<const C*>(Bb + o) : C{T(0), T(0)};
   }
   if (MODE == kModeCross) {
 #pragma unroll

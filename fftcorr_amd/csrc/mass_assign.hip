@@ -11,11 +11,11 @@
 //   2. stable radix sort of (key, particle index): within a key cell the
 //      particles keep their input order;
 //   3. k_permute      records in sorted order;
-//   4. k_cell_runs    per occupied key cell, its first sorted particle and
-//                     its count, and its bit in an occupancy bitmap (cleared
-//                     first; the count table is read only where the bit is
-//                     set, so it needs no clearing);
-//   5. k_ma_tile      one workgroup per 4 x 8 x 32 tile of target cells lists
+//   4. k_cell_runs    per occupied key cell, its run {first sorted particle,
+//                     count} (one 8-byte entry) and its bit in an occupancy
+//                     bitmap (cleared first; the run table is read only where
+//                     the bit is set, so it needs no clearing);
+//   5. k_ma_tile      one workgroup per 4 x 4 x 32 tile of target cells lists
 //                     the particles of every key cell whose stencil reaches
 //                     the tile, in key-cell order; each wave takes a contiguous
 //                     quarter of the list, 64 particles at a time, forms their
@@ -40,7 +40,10 @@ namespace fc {
 namespace {
 
 constexpr int kPrepThreads = 256;
-constexpr int TX = 4, TY = 8, TZ = 32;  // target tile of k_ma_tile
+#ifndef FC_MA_TILE_Y
+#define FC_MA_TILE_Y 4
+#endif
+constexpr int TX = 4, TY = FC_MA_TILE_Y, TZ = 32;  // target tile of k_ma_tile
 constexpr int kTileCells = TX * TY * TZ;
 constexpr int kTileThreads = 256;
 constexpr int kTileWaves = kTileThreads / kWave;
@@ -179,13 +182,12 @@ __global__ void k_zero_i32(int32_t* __restrict__ p, int64_t n) {
 }
 
 // Runs of equal keys in the sorted keys[0, nvalid): the first particle of
-// each run writes beg[key] = its position and cnt[key] = the run length,
+// each run writes runs[key] = {its position, the run length}, the length
 // found by an exponential then a binary search (a long run costs its first
 // thread O(log length), not O(length)), and sets the key's occupancy bit.
 __global__ void k_cell_runs(const uint32_t* __restrict__ keys,
                             const unsigned long long* __restrict__ nvalid_p,
-                            int32_t* __restrict__ beg, int32_t* __restrict__ cnt,
-                            uint32_t* __restrict__ occ) {
+                            int2* __restrict__ runs, uint32_t* __restrict__ occ) {
   const int64_t nvalid = (int64_t)*nvalid_p;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvalid;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -204,16 +206,15 @@ __global__ void k_cell_runs(const uint32_t* __restrict__ keys,
       else
         hi = mid;
     }
-    beg[k] = (int32_t)i;
-    cnt[k] = (int32_t)(lo + 1 - i);
+    runs[k] = make_int2((int)i, (int)(lo + 1 - i));
     atomicOr(&occ[k >> 5], 1u << (k & 31));
   }
 }
 
-// Particles in key cell c: 0 unless its occupancy bit is set.
-__device__ __forceinline__ int cell_count(const uint32_t* __restrict__ occ,
-                                          const int32_t* __restrict__ cnt, int64_t c) {
-  return ((occ[c >> 5] >> (c & 31)) & 1u) ? cnt[c] : 0;
+// Run {first, count} of key cell c: {0, 0} unless its occupancy bit is set.
+__device__ __forceinline__ int2 cell_run(const uint32_t* __restrict__ occ,
+                                         const int2* __restrict__ runs, int64_t c) {
+  return ((occ[c >> 5] >> (c & 31)) & 1u) ? runs[c] : make_int2(0, 0);
 }
 
 // v mod n for the halo of a tile: one compare in the common case, the full
@@ -259,9 +260,9 @@ struct Stencil<3> {  // CIC about cell centres: key cell a <= x - 1/2 < a + 1
 
 template <typename Tg, int SCHEME>
 __global__ void __launch_bounds__(kTileThreads)
-    k_ma_tile(Tg* __restrict__ grid, GridGeom g, const int32_t* __restrict__ beg,
-              const int32_t* __restrict__ cnt, const uint32_t* __restrict__ occ,
-              const double4* __restrict__ srec, int tiles_y, int tiles_z) {
+    k_ma_tile(Tg* __restrict__ grid, GridGeom g, const int2* __restrict__ runs,
+              const uint32_t* __restrict__ occ, const double4* __restrict__ srec, int tiles_y,
+              int tiles_z) {
 #pragma clang fp contract(off)
   using S = Stencil<SCHEME>;
   constexpr int W = S::OMAX - S::OMIN + 1;
@@ -303,19 +304,18 @@ __global__ void __launch_bounds__(kTileThreads)
         word[q] = occ[cell[q] >> 5];
       }
     }
-    int m[kPer], b0[kPer];
+    int2 run[kPer];
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       const bool on = cell[q] >= 0 && ((word[q] >> (cell[q] & 31)) & 1u);
-      m[q] = on ? cnt[cell[q]] : 0;
-      b0[q] = on ? beg[cell[q]] : 0;
+      run[q] = on ? runs[cell[q]] : make_int2(0, 0);
     }
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       const int rc = tid + q * kTileThreads;
       if (rc < RC) {
-        pre[rc] = m[q];
-        bg[rc] = b0[q];
+        pre[rc] = run[q].y;
+        bg[rc] = run[q].x;
       }
     }
   }
@@ -459,8 +459,8 @@ constexpr int DX = WTX + WC - 1, DY = WTY + WC - 1, DZ = WTZ + WC - 1;
 
 template <typename Tg>
 __global__ void __launch_bounds__(WTX* WTY* WTZ)
-    k_ma_gather_wavelet(Tg* __restrict__ grid, GridGeom g, const int32_t* __restrict__ beg,
-                        const int32_t* __restrict__ cnt, const uint32_t* __restrict__ occ,
+    k_ma_gather_wavelet(Tg* __restrict__ grid, GridGeom g, const int2* __restrict__ runs,
+                        const uint32_t* __restrict__ occ,
                         const double4* __restrict__ srec, const double* __restrict__ table,
                         int tiles_y, int tiles_z) {
 #pragma clang fp contract(off)
@@ -480,9 +480,9 @@ __global__ void __launch_bounds__(WTX* WTY* WTZ)
     const int gy = wrap_index(y0 + hy - (WC - 2), g.n1);
     const int gz = wrap_index(z0 + hz - (WC - 2), g.n2);
     const int64_t c = ((int64_t)gx * g.n1 + gy) * g.n2 + gz;
-    const int m = cell_count(occ, cnt, c);
-    cs[h] = m > 0 ? beg[c] : 0;
-    ce[h] = cs[h] + m;
+    const int2 r = cell_run(occ, runs, c);
+    cs[h] = r.x;
+    ce[h] = r.x + r.y;
   }
   __syncthreads();
   const int tz = threadIdx.x % WTZ, ty = (threadIdx.x / WTZ) % WTY, tx = threadIdx.x / (WTZ * WTY);
@@ -532,8 +532,8 @@ const double* wavelet_on_device(fc_ctx* ctx) {
 }
 
 template <typename Tg>
-void launch_gather(int scheme, Tg* grid, const GridGeom& gg, const int32_t* beg,
-                   const int32_t* cnt, const uint32_t* occ, const double4* srec, hipStream_t s,
+void launch_gather(int scheme, Tg* grid, const GridGeom& gg, const int2* runs,
+                   const uint32_t* occ, const double4* srec, hipStream_t s,
                    const double* table = nullptr) {
   const bool wav = scheme == 2;
   const int bx = wav ? WTX : TX, by = wav ? WTY : TY, bz = wav ? WTZ : TZ;
@@ -543,20 +543,17 @@ void launch_gather(int scheme, Tg* grid, const GridGeom& gg, const int32_t* beg,
   dim3 grd((unsigned)nb), blk(wav ? WTX * WTY * WTZ : kTileThreads);
   switch (scheme) {
     case 0:
-      hipLaunchKernelGGL((k_ma_tile<Tg, 0>), grd, blk, 0, s, grid, gg, beg, cnt, occ, srec, ty,
-                         tz);
+      hipLaunchKernelGGL((k_ma_tile<Tg, 0>), grd, blk, 0, s, grid, gg, runs, occ, srec, ty, tz);
       break;
     case 1:
-      hipLaunchKernelGGL((k_ma_tile<Tg, 1>), grd, blk, 0, s, grid, gg, beg, cnt, occ, srec, ty,
-                         tz);
+      hipLaunchKernelGGL((k_ma_tile<Tg, 1>), grd, blk, 0, s, grid, gg, runs, occ, srec, ty, tz);
       break;
     case 3:
-      hipLaunchKernelGGL((k_ma_tile<Tg, 3>), grd, blk, 0, s, grid, gg, beg, cnt, occ, srec, ty,
-                         tz);
+      hipLaunchKernelGGL((k_ma_tile<Tg, 3>), grd, blk, 0, s, grid, gg, runs, occ, srec, ty, tz);
       break;
     case 2:
-      hipLaunchKernelGGL((k_ma_gather_wavelet<Tg>), grd, blk, 0, s, grid, gg, beg, cnt, occ,
-                         srec, table, ty, tz);
+      hipLaunchKernelGGL((k_ma_gather_wavelet<Tg>), grd, blk, 0, s, grid, gg, runs, occ, srec,
+                         table, ty, tz);
       break;
   }
   FC_LAUNCH_CHECK();
@@ -613,12 +610,10 @@ void mass_assign(fc_grid* grid, const double* pos_dev, int64_t n, int ncol,
   sc[7].reserve(sizeof(double) * 2 * (nbp + 1));
   sc[8].reserve(sizeof(unsigned long long) * (nbp + 1));
   const uint64_t nwords = (ncells + 31) / 32;
-  sc[9].reserve(sizeof(int32_t) * ncells);
-  sc[10].reserve(sizeof(int32_t) * ncells);
+  sc[9].reserve(sizeof(int2) * ncells);
   sc[11].reserve(sizeof(uint32_t) * nwords);
   DeviceBuffer &keys = sc[1], &keys2 = sc[2], &idx = sc[3], &idx2 = sc[4], &rec = sc[5],
-               &srec = sc[6], &partw = sc[7], &partn = sc[8], &beg = sc[9], &cnt = sc[10],
-               &occ = sc[11];
+               &srec = sc[6], &partw = sc[7], &partn = sc[8], &runs = sc[9], &occ = sc[11];
   const double anchor = scheme == 3 ? 0.5 : 0.0;  // CIC keys by its 2x2x2 block
 
   hipLaunchKernelGGL(k_ma_prepare, dim3(nbp), dim3(kPrepThreads), 0, s, pos_dev, n, ncol, w_dev,
@@ -649,16 +644,16 @@ void mass_assign(fc_grid* grid, const double* pos_dev, int64_t n, int ncol,
                      occ.as<int32_t>(), (int64_t)nwords);
   FC_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_cell_runs, dim3(grid1d(n, 256, 65536)), dim3(256), 0, s,
-                     keys2.as<uint32_t>(), partn.as<unsigned long long>() + nbp,
-                     beg.as<int32_t>(), cnt.as<int32_t>(), occ.as<uint32_t>());
+                     keys2.as<uint32_t>(), partn.as<unsigned long long>() + nbp, runs.as<int2>(),
+                     occ.as<uint32_t>());
   FC_LAUNCH_CHECK();
   FC_HIP(hipEventRecord(ev[1], s));
   if (grid->prec == FC_FP64)
-    launch_gather<double>(scheme, grid->data.as<double>(), gg, beg.as<int32_t>(),
-                          cnt.as<int32_t>(), occ.as<uint32_t>(), srec.as<double4>(), s, wtab);
+    launch_gather<double>(scheme, grid->data.as<double>(), gg, runs.as<int2>(),
+                          occ.as<uint32_t>(), srec.as<double4>(), s, wtab);
   else
-    launch_gather<float>(scheme, grid->data.as<float>(), gg, beg.as<int32_t>(),
-                         cnt.as<int32_t>(), occ.as<uint32_t>(), srec.as<double4>(), s, wtab);
+    launch_gather<float>(scheme, grid->data.as<float>(), gg, runs.as<int2>(),
+                         occ.as<uint32_t>(), srec.as<double4>(), s, wtab);
   FC_HIP(hipEventRecord(ev[2], s));
 
   double wsum[2];
@@ -731,8 +726,8 @@ int fc_mass_assign_footprint(fc_grid* grid, int64_t n, int ncol, int has_weights
                                sizeof(double4) * n,
                                sizeof(double) * 2 * (nbp + 1),
                                sizeof(unsigned long long) * (nbp + 1),
-                               sizeof(int32_t) * ncells,
-                               sizeof(int32_t) * ncells,
+                               sizeof(int2) * ncells,
+                               0,
                                sizeof(uint32_t) * ((ncells + 31) / 32)};
       for (int i = 0; i < 12; ++i)
         if (want[i] > ctx->scratch[i].bytes()) g += want[i] - ctx->scratch[i].bytes();

@@ -65,10 +65,20 @@ int fc_device_count(int* count);
 /* Free and total device memory in bytes (admission control of the serving
  * layer; no reference counterpart). */
 int fc_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
+/* Device memory held by this library on `device` (every handle's and
+ * context's buffers) and its peak since the last reset; the serving layer
+ * admits jobs against it (no reference counterpart). */
+int fc_device_bytes(int device, size_t* in_use, size_t* peak);
+int fc_device_bytes_reset_peak(int device);
 /* One device + one HIP stream; all handles made from it run on that stream. */
 int fc_ctx_create(int device, fc_ctx** out);
 int fc_ctx_destroy(fc_ctx* ctx);
 int fc_ctx_synchronize(fc_ctx* ctx);
+/* Device bytes held by the context's reused work buffers (mass assignment),
+ * and their release (after the context's pending work; later calls grow them
+ * again).  No reference counterpart. */
+int fc_ctx_footprint(fc_ctx* ctx, size_t* bytes);
+int fc_ctx_release_scratch(fc_ctx* ctx);
 
 /* ---- ConfigSpaceGrid: cc/grid/config_space_grid.h:10-56,
  *      fftcorr/grid/config_space_grid.pyx:11-207 -------------------------- */
@@ -116,6 +126,12 @@ int fc_set_wavelet_table(const double* table, int n);
 int fc_mass_assign(fc_grid* grid, const double* pos, int64_t n, int ncol,
                    const double* weights, double w_scalar, int scheme,
                    int periodic_wrap, double shift, fc_ma_stats* stats);
+/* Device memory a call of n particles on this grid needs beyond what its
+ * context holds: *grow = growth of the context's reused work buffers (kept
+ * after the call), *transient = staging of host inputs (host_input = 1, as
+ * fc_mass_assign; freed on return).  No reference counterpart. */
+int fc_mass_assign_footprint(fc_grid* grid, int64_t n, int ncol, int has_weights,
+                             int host_input, size_t* grow, size_t* transient);
 /* Same, with pos/weights already in device memory (HBM-resident catalogs). */
 int fc_mass_assign_device(fc_grid* grid, const double* pos_dev, int64_t n, int ncol,
                           const double* weights_dev, double w_scalar, int scheme,
@@ -157,6 +173,10 @@ int fc_corr_create(fc_ctx* ctx, int kind, const int shape[3], double cell_size,
                    const double posmin[3], int window_correct, double rmax, double dr,
                    double kmax, double dk, int maxell, int precision, fc_corr** out);
 int fc_corr_destroy(fc_corr* corr);
+/* Device bytes the correlator holds now (*held) and will hold once its
+ * correlation path has run (*full; interlaced = 1 for
+ * fc_corr_autocorrelate_interlaced).  No reference counterpart. */
+int fc_corr_footprint(fc_corr* corr, int interlaced, size_t* held, size_t* full);
 /* Geometry of the outputs: r/k bin counts, number of multipoles, subgrids. */
 int fc_corr_geometry(fc_corr* corr, int* n_rbins, int* n_kbins, int* n_ell,
                      int rshape[3], int kshape[3]);
@@ -209,8 +229,10 @@ int fc_corr_stage_times(fc_corr* corr, double* ms, int64_t* launches);
  *      reference correlates one grid per call, correlator.pyx:72-184; this
  *      runs many same-geometry mocks per launch) ---------------------------- */
 /* nbatch member grids [shape] stored back to back on the device, one shared
- * PeriodicCorrelator geometry (arguments as fc_corr_create).  Needs shape[0]
- * a power of two from 8 to 512. */
+ * PeriodicCorrelator geometry (arguments as fc_corr_create).  Every pass is
+ * one launch over all members, except the x pass of lengths the fused x
+ * kernel does not take (not a power of two from 8 to 512, e.g. 384), which
+ * runs member by member. */
 int fc_batch_create(fc_ctx* ctx, const int shape[3], double cell_size, int window_correct,
                     double rmax, double dr, double kmax, double dk, int maxell, int precision,
                     int nbatch, fc_batch** out);

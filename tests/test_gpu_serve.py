@@ -71,20 +71,42 @@ def test_job_equals_direct_path():
     _same(got, c.results())
 
 
-def test_budget_serialises_admission_and_rejects_oversized_jobs():
-    jobs = [Job(n=48, box=BOX, positions=uniform_catalog(20000, 0.0, BOX, 70 + i), tag=i,
-                rmax=40.0, dr=5.0, kmax=0.3, dk=0.02) for i in range(6)]
+def test_admission_keeps_measured_device_bytes_within_budget():
+    """The library's own count of the device bytes it holds (every buffer of
+    every handle and context) never rises above the budget while a stream
+    of jobs of three geometries runs on four slots."""
+    from fftcorr_amd import _lib
+    jobs = [Job(n=n, box=BOX, positions=uniform_catalog(20000 + 5000 * i, 0.0, BOX, 70 + i),
+                tag=i, rmax=40.0, dr=5.0, kmax=0.3, dk=0.02)
+            for i, n in enumerate([48, 32, 64, 48, 64, 32, 48, 64])]
     need = max(job_bytes(j, "mixed") for j in jobs)
-    with CorrelationServer(slots=4, hbm_budget=int(1.5 * need)) as srv:
+    budget = int(1.5 * need)
+    base, _ = _lib.device_bytes()
+    _lib.reset_device_peak()
+    with CorrelationServer(slots=4, hbm_budget=budget) as srv:
         res = [f.result(timeout=120) for f in [srv.submit(j) for j in jobs]]
-    spans = sorted((r.started, r.finished) for r in res)
-    # Two of these jobs do not fit the budget together: one runs at a time
-    # (a job starts only after the previous one released its bytes).
-    for (s0, f0), (s1, f1) in zip(spans, spans[1:]):
-        assert s1 >= f0 - 1e-3
+        _, peak = _lib.device_bytes()
+    assert peak - base <= budget, (peak - base, budget)
+    assert sorted(r.tag for r in res) == list(range(len(jobs)))
     with CorrelationServer(slots=1, hbm_budget=1000) as srv:
         with pytest.raises(MemoryError):
             srv.submit(jobs[0]).result(timeout=10)
+
+
+@pytest.mark.parametrize("n", [16, 48, 64])
+def test_footprint_estimates_bound_measured_bytes(n):
+    """The admission estimates (state_bytes, assign_bytes) bound what a job's
+    geometry and mass assignment then hold, as the library measures them."""
+    from fftcorr_amd.serve import assign_bytes, state_bytes
+    job = Job(n=n, box=BOX, positions=thomas_catalog(400, 50, 4.0, 0.0, BOX, n),
+              rmax=60.0, dr=5.0, kmax=0.4, dk=0.02)
+    for prec in ("mixed", "fp64"):
+        with CorrelationServer(slots=1, precision=prec) as srv:
+            srv.submit(job).result(timeout=120)
+            slot = srv._slots[0]
+            held = slot.state[slot.key(job)][2]
+            assert 0 < held <= state_bytes(job, prec), (held, state_bytes(job, prec))
+            assert 0 < slot.ctx_bytes <= assign_bytes(job)
 
 
 def test_poisson_stream_is_seeded():
