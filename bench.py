@@ -75,21 +75,21 @@ def make_grids(lib, ctx, pos, cfg, precision):
     return grids, ma_ms, st
 
 
-def stage_bytes(cfg, s_work, s_grid, kshape, rs, s_inv=None):
+def stage_bytes(cfg, s_work, s_grid, kshape, rs, s_inv=None, k_entries=None):
     """Algorithmic HBM bytes per launch of each stage: the data each stage
     must read and write once.  Complex half grid n^2 (n/2+1) elements of
     2 s_work bytes; real grid n^3 of s_grid; compact k-subgrid values
     ks0 ks1 (ks2/2+1) of s_work; pruned inverse arrays rs n (n/2+1) and
-    rs^2 (n/2+1) complex and the r subgrid rs^3 real, of s_inv bytes.  The
-    histograms read, per subgrid cell, its 4-byte cell index and its value
-    (k: one s_work value; r: one s_inv value and three FP64 multipole
-    weights)."""
+    rs^2 (n/2+1) complex and the r subgrid rs^3 real, of s_inv bytes.  The k
+    histogram reads, per compact k value inside kmax (the half subgrid
+    kz >= 0, whose mirror cells share the values), its 4-byte index and the
+    s_work value; the r histogram reads, per subgrid cell, its 4-byte cell
+    index, its s_inv value and three FP64 multipole weights."""
     s_inv = s_inv or s_work
     n = cfg["n"]
     nh = n // 2 + 1
     half = n * n * nh * 2 * s_work
     pk = kshape[0] * kshape[1] * (kshape[2] // 2 + 1) * s_work
-    K = kshape[0] * kshape[1] * kshape[2]
     q1 = rs * n * nh * 2 * s_inv
     q2 = rs * rs * nh * 2 * s_inv
     xs = rs ** 3 * s_inv
@@ -97,11 +97,28 @@ def stage_bytes(cfg, s_work, s_grid, kshape, rs, s_inv=None):
         "fft_z_fwd": n ** 3 * s_grid + half,
         "fft_y_fwd": 2 * half,
         "combine": 2 * half + pk + q1,  # fused x pass: read a, b; write pk, pruned q1
-        "hist_k": K * (4 + s_work),
+        "hist_k": (k_entries if k_entries is not None else pk // s_work) * (4 + s_work),
         "fft_y_inv": q1 + q2,
         "fft_z_inv": q2 + xs,
         "hist_r": rs ** 3 * (4 + s_inv + 3 * 8),
     }
+
+
+def k_half_entries(cfg, kshape):
+    """Entries of the k histogram: compact k values (half subgrid kz >= 0)
+    whose |k| falls in a bin, with the correlator's FP64 axes and bin rule."""
+    n = cfg["n"]
+    knyq = np.pi / (cfg["box"] / n)
+    step = 2.0 * knyq / n
+    ax = [(-(ks // 2)) * 2.0 * knyq / n + np.arange(ks) * step for ks in kshape]
+    kz = ax[2][kshape[2] // 2:]
+    nb = np.floor(cfg["kmax"] / cfg["dk"])
+    xy = ax[0][:, None] * ax[0][:, None] + ax[1][None, :] * ax[1][None, :]
+    count = 0
+    for z in kz:
+        f = np.floor(np.sqrt(xy + z * z) / cfg["dk"])
+        count += int(np.count_nonzero((f >= 0) & (f < nb)))
+    return count
 
 
 def run_form(lib, corr_mod, ctx, pos, cfg, precision, steps, warmup, profile,
@@ -485,6 +502,43 @@ BATCH512_CFG = dict(n=512, box=2000.0, parents=80_000, children=100, sigma=3.0, 
                     rmax=200.0, dr=5.0, kmax=0.5, dk=0.005, maxell=4, window=3, scheme=3, reps=3)
 
 
+def autotune_512(lib, corr_mod, ctx, pos, cfg, steps, default_out):
+    """The autotuner (the counterpart of the reference's FFTW_MEASURE
+    planning) on the 512^3 plans: its radix plan and tile choices and best
+    pass times, and the headline form re-timed with the tuned plan against
+    the default plan it was measured with."""
+    from fftcorr_amd import fftw
+    from fftcorr_amd.autotune import autotune
+    n = cfg["n"]
+    shape = (n, n, n)
+    t0 = time.perf_counter()
+    res = autotune(shape, "mixed", reps=5)
+    tune_s = time.perf_counter() - t0
+    res64 = autotune(shape, "fp64", reps=5)
+    out, g, c = run_form(lib, corr_mod, ctx, pos, cfg, "mixed", steps, 3, profile=True)
+    del g, c
+    fftw.forget_wisdom()
+    d_ms = 1e3 * default_out["wall_s"] / default_out["steps"]
+    t_ms = 1e3 * out["wall_s"] / steps
+    err = max(float(floor_err(out["res"][k], default_out["res"][k]).max())
+              for k in ("k_hist", "r_hist"))
+    return {
+        "candidates": "z pass: radix plans 8x8x4 and 16x16, rows per workgroup 1-64; y and x "
+                      "passes: radix plans 8x8x8 and 16x16x2, columns per workgroup 1-32; "
+                      "z+y chunk sizes",
+        "mixed": {"choices": {k: v for k, v in res.items() if k != "ms"},
+                  "best_pass_ms": res["ms"]},
+        "fp64": {"choices": {k: v for k, v in res64.items() if k != "ms"},
+                 "best_pass_ms": res64["ms"]},
+        "tune_s": tune_s,
+        "default_ms_per_step": d_ms,
+        "tuned_ms_per_step": t_ms,
+        "tuned_over_default": d_ms / t_ms,
+        "tuned_stage_ms": {k: v[0] / v[1] for k, v in out["stages"].items() if v[1]},
+        "max_floor_rel_err_tuned_vs_default": err,
+    }
+
+
 def batched_512(lib, corr_mod, ctx, pos0):
     """The north-star form: a batch of 8 periodic 512^3 correlations (l <= 4,
     kmax 0.5 dk 0.005, rmax 200 dr 5, CIC, sinc^4 window; the headline
@@ -616,7 +670,8 @@ def main():
                          for k in ("k_hist", "r_hist"))
 
     # Roofline of the dominant stage (HIP events on the correlator's stream).
-    sb = stage_bytes(cfg, 4, 4, fm["kshape"], fm["rshape"][0], s_inv=8)
+    sb = stage_bytes(cfg, 4, 4, fm["kshape"], fm["rshape"][0], s_inv=8,
+                     k_entries=k_half_entries(cfg, fm["kshape"]))
     st = {k: v for k, v in fm["stages"].items() if v[1] > 0}
     dom = max(st, key=lambda k: st[k][0])
     avg_ms = st[dom][0] / st[dom][1]
@@ -656,6 +711,8 @@ def main():
         log("config 1: 128^3 CIC xi_0 FP64, GPU and CPU")
         c1 = config1(lib, corr_mod, ctx, 20)
 
+    log("autotuner on the 512^3 plans; headline form re-timed with the tuned plan")
+    tune_report = autotune_512(lib, corr_mod, ctx, pos, cfg, args.steps, fm)
     log("Yamamoto correlator at 512^3 (anisotropic stress case)")
     yama_report = yamamoto_512(lib, corr_mod, ctx, pos)
     log("batched mocks: 64 x 256^3 TSC, mixed, HIP graph")
@@ -719,6 +776,7 @@ def main():
         "yamamoto_512": yama_report,
         "batched_256": batch_report,
         "batched_512": batch512_report,
+        "autotune_512": tune_report,
         "config1_128": c1,
         "serving": serve_report,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,

@@ -133,6 +133,8 @@ SIGNATURES = {
     "fc_batch_correlator": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "fc_batch_last_time_ms": (ctypes.c_int, [_vp, _dp]),
     "fc_plan_tiles": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _ip]),
+    "fc_plan_variants": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _ip]),
+    "fc_fft_radices": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _ip, _ip]),
     "fc_corr_stage_times": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "fc_hist_accumulate": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                           _dp, _dp, _i64, _ip, _dp]),

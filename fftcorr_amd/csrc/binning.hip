@@ -9,9 +9,14 @@ namespace fc {
 void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
                         const std::vector<double>& ay, const std::vector<double>& az,
                         double binsize, int nbins, int64_t track_cell,
-                        const std::function<uint64_t(uint32_t)>* order) {
+                        const std::function<uint64_t(uint32_t)>* order,
+                        std::vector<int16_t>* cell_bins) {
   const int s0 = (int)ax.size(), s1 = (int)ay.size(), s2 = (int)az.size();
   if ((uint64_t)s0 * s1 * s2 > 0xffffffffULL) fail(FC_ERR_INVALID, "subgrid too large");
+  if (cell_bins) {
+    if (nbins >= 32768) fail(FC_ERR_RUNTIME, "internal: too many bins for 16-bit cell bins");
+    cell_bins->assign((size_t)s0 * s1 * s2, (int16_t)-1);
+  }
   // Bin of cell (i, j, k): floor((|r| - 0) / binsize) with
   // |r| = sqrt((x*x + y*y) + z*z) in FP64 (correlator.h:129-135, :180-186;
   // histogram_list.h:57).  Host x86-64 code without FMA, like the reference.
@@ -40,6 +45,7 @@ void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
             for (int k = 0; k < s2; ++k) {
               const int b = bin_of(i, j, k);
               if (b >= 0) ++cnt[t][b];
+              if (cell_bins) (*cell_bins)[((size_t)i * s1 + j) * s2 + k] = (int16_t)b;
             }
       });
     for (auto& x : th) x.join();

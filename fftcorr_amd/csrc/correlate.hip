@@ -20,6 +20,7 @@
 // indices and every sum stay in FP64.
 #include <algorithm>
 #include <array>
+#include <thread>
 #include <type_traits>
 
 #include "fc_objects.h"
@@ -47,6 +48,10 @@ struct fc_corr {
   std::vector<double> h_rx, h_kx, h_ky, h_kz;
   fc::DeviceBuffer rx, kx, ky, kz, wx, wy, wz;  // FP64 axes and per-axis window W^2
   fc::BinnedCells kbin, rbin;                   // subgrid cells sorted by bin
+  // Periodic k histogram over the compact values (KEntryHalf): their
+  // indices sorted by bin, when the geometry allows it (khalf_ok).
+  fc::BinnedCells khalf;
+  bool khalf_ok = false;
   fc::DeviceBuffer rw;                          // periodic: r weights per entry
   double kcoef[3] = {0, 0, 0};                  // periodic k-space Y_l0 coefficients
   // FFT plan + work arrays (complex rows of plan->ldc).
@@ -308,6 +313,40 @@ struct KEntryLm {
   }
 };
 
+// Periodic k space over the half of the k subgrid that the compact values
+// cover: entry e is the compact value pk[e] = pk[sx][sy][c] itself (the cell
+// (sx, sy, oz + c)).  A value with c > 0 is also read by the mirror cell
+// (ks0-1-sx, ks1-1-sy, oz-c); with odd subgrid axes the two have the same |k|
+// up to rounding and the even multipoles weight them alike, so the value
+// enters once with weight 2 (the host uses these entries only when every
+// mirror pair shares its bin).  Half the entries of KEntry, each value
+// fetched once.
+template <typename T, int NV>
+struct KEntryHalf {
+  const uint32_t* cells;  // compact indices, sorted by bin
+  const T* pk;
+  FastDiv dz, d1;  // division by oz + 1 and ks1
+  int nz, ks1;
+  const double *kx, *ky, *kz, *wx, *wy, *wz;  // kz, wz from the kz = 0 plane on
+  double coef[NV];
+  int64_t bstride = 0;  // pk elements per batch member
+  __device__ __forceinline__ void offset(int m) { pk += m * bstride; }
+  __device__ __forceinline__ void operator()(int64_t pos, double* v) const {
+#pragma clang fp contract(off)
+    const uint32_t e = cells[pos];
+    const uint32_t t = dz.div(e);
+    const int c = (int)(e - t * (uint32_t)nz);
+    const uint32_t iu = d1.div(t);
+    const int j = (int)(t - iu * (uint32_t)ks1), i = (int)iu;
+    const double x = (double)pk[e];
+    const double invw = 1.0 / (wx[i] * wy[j] * wz[c]);
+    const double mult = c > 0 ? 2.0 : 1.0;
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+      v[q] = (ylm_eval(2 * q, 0, kx[i], ky[j], kz[c], coef[q], invw) * x) * mult;
+  }
+};
+
 // Precomputed weights wt[q][pos] (periodic r space).
 template <typename V, int NV>
 struct WEntry {
@@ -519,6 +558,39 @@ double window_sq(int kind, double k, double cell) {
   return 0;
 }
 
+// Periodic k histogram over the compact values (KEntryHalf): allowed when
+// the k subgrid axes are odd (every compact value with c > 0 has its mirror
+// cell) and every mirror pair shares its bin (cb: the bin of every k cell).
+// The entries are then the compact indices of the half subgrid kz >= 0,
+// sorted by bin and in index order within a bin; the counts stay those of
+// the whole subgrid.
+void setup_khalf(fc_corr* c, const std::vector<int16_t>& cb) {
+  c->khalf_ok = false;
+  const int s0 = c->kshape[0], s1 = c->kshape[1], s2 = c->kshape[2];
+  if (c->kind != FC_PERIODIC || s0 % 2 == 0 || s1 % 2 == 0 || s2 % 2 == 0) return;
+  const int oz = s2 / 2;
+  const int64_t ncols = (int64_t)s0 * s1;
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<char> ok(nt, 1);
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      for (int64_t col = (int64_t)t; col < ncols; col += nt) {
+        const int a = (int)(col / s1), b = (int)(col % s1);
+        const size_t row = ((size_t)a * s1 + b) * s2;
+        const size_t mrow = ((size_t)(s0 - 1 - a) * s1 + (s1 - 1 - b)) * s2;
+        for (int k = 1; k <= oz; ++k)
+          if (cb[mrow + oz - k] != cb[row + oz + k]) ok[t] = 0;
+      }
+    });
+  for (auto& x : th) x.join();
+  for (char o : ok)
+    if (!o) return;
+  const std::vector<double> kz_half(c->h_kz.begin() + oz, c->h_kz.end());
+  build_binned_cells(c->khalf, c->h_kx, c->h_ky, kz_half, c->dk, c->nkbins);
+  c->khalf_ok = true;
+}
+
 void setup_geometry(fc_corr* c) {
   // HistogramList(maxell/2+1, 0, max, binsize), histogram_list.h:13-25.
   auto hist_bins = [](double maxval, double binsize, const char* what) {
@@ -568,7 +640,13 @@ void setup_geometry(fc_corr* c) {
     return (uint64_t)(k >= oz ? ks.at(i, j, k - oz)
                               : ks.at(ks.ks0 - 1 - i, ks.ks1 - 1 - j, oz - k));
   };
-  build_binned_cells(c->kbin, c->h_kx, c->h_ky, c->h_kz, c->dk, c->nkbins, -1, &kkey);
+  {
+    std::vector<int16_t> cell_bins;
+    const bool want = c->kind == FC_PERIODIC && c->nkbins < 32768;
+    build_binned_cells(c->kbin, c->h_kx, c->h_ky, c->h_kz, c->dk, c->nkbins, -1, &kkey,
+                       want ? &cell_bins : nullptr);
+    if (want) setup_khalf(c, cell_bins);
+  }
   const int64_t o = c->rmax_cells;
   build_binned_cells(c->rbin, c->h_rx, c->h_rx, c->h_rx, c->dr, c->nrbins,
                      (o * c->rshape + o) * c->rshape + o);
@@ -645,7 +723,7 @@ void forward_zy(fc_corr* c, fc_grid* g, cplx<T>* dst, Loader ld) {
     ColView<T> v = y_view<T>(p, d);
     v.A = nx;
     launch_cols_wide<T, -1, A>(p.n[1], v, p.tile[1], pa.tw1.as<cplx<A>>(), p.tw1.as<cplx<T>>(),
-                               s);
+                               s, p.var[1]);
   }
 }
 
@@ -655,7 +733,7 @@ void forward_x(fc_corr* c, cplx<T>* dst) {
   Stage st(c, FC_STAGE_FFT_X_FWD);
   launch_cols_wide<T, -1, A>(p.n[0], x_view<T>(p, dst), p.tile[0],
                              c->plan_inv->tw0.as<cplx<A>>(), p.tw0.as<cplx<T>>(),
-                             c->ctx->stream);
+                             c->ctx->stream, p.var[0]);
 }
 
 // Full forward 3D FFT.
@@ -685,6 +763,24 @@ void ensure_phase_tables(fc_corr* c) {
 }
 
 KSlots kslots(const fc_corr* c) { return KSlots{c->kshape[0], c->kshape[1], c->kshape[2]}; }
+
+// k histogram of nmem members' compact values (member m at pk + m * bstride)
+// over the half-subgrid entries; hist / counts rows of member m at
+// m * out_bstride.
+template <typename T, int NV>
+void run_hist_khalf(fc_corr* c, const T* pk, int64_t bstride, DeviceBuffer& part, double* hist,
+                    int* counts, int row0, int crow0, int crow1, hipStream_t s, int nmem = 1,
+                    int64_t out_bstride = 0) {
+  const int nz = c->kshape[2] / 2 + 1;
+  const int oz = c->kshape[2] / 2;
+  KEntryHalf<T, NV> ke{c->khalf.cells.as<uint32_t>(), pk, FastDiv((uint32_t)nz),
+                       FastDiv((uint32_t)c->kshape[1]), nz, c->kshape[1], c->kx.as<double>(),
+                       c->ky.as<double>(), c->kz.as<double>() + oz, c->wx.as<double>(),
+                       c->wy.as<double>(), c->wz.as<double>() + oz, {}, bstride};
+  for (int q = 0; q < NV; ++q) ke.coef[q] = c->kcoef[q];
+  run_hist_seg<NV>(ke, c->khalf, part, hist, counts, row0, crow0, crow1, s, -1, nullptr, nmem,
+                   out_bstride, c->kbin.counts.as<int>());
+}
 
 template <typename T, typename TI>
 void ensure_periodic_buffers(fc_corr* c) {
@@ -862,14 +958,14 @@ void power_and_pruned_inverse(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode
     Stage st(c, FC_STAGE_FFT_X_INV);
     ColView<TI, T> v{w, 0, (int64_t)p.n[1] * p.ldc, (int64_t)p.ldc, c->q1.as<cplx<TI>>(), 0,
                      (int64_t)p.n[1] * ldq, ldq, 1, p.n[1], p.n2h, rs, o};
-    launch_cols<TI, +1, T>(p.n[0], v, pi.tile[0], pi.tw0.as<cplx<TI>>(), s);
+    launch_cols<TI, +1, T>(p.n[0], v, pi.tile[0], pi.tw0.as<cplx<TI>>(), s, pi.var[0]);
   }
   // Pruned inverse y pass: q1[sx][y][c] -> q2[sx][sy][c], only the rs lags.
   {
     Stage st(c, FC_STAGE_FFT_Y_INV);
     ColView<TI> v{c->q1.as<cplx<TI>>(), (int64_t)pi.n[1] * ldq, ldq, 0, c->q2.as<cplx<TI>>(),
                   (int64_t)rs * ldq, ldq, 0, rs, 1, pi.n2h, rs, o};
-    launch_cols<TI, +1>(pi.n[1], v, pi.tile[1], pi.tw1.as<cplx<TI>>(), s);
+    launch_cols<TI, +1>(pi.n[1], v, pi.tile[1], pi.tw1.as<cplx<TI>>(), s, pi.var[1]);
   }
   // Pruned z C2R: rows of q2 -> r subgrid rows xi[sx][sy][sz].
   Stage st(c, FC_STAGE_FFT_Z_INV);
@@ -909,13 +1005,18 @@ void periodic_hists(fc_corr* c) {
   ensure_periodic_weights<NV>(c);
   {
     Stage st(c, FC_STAGE_HIST_K);
-    KEntry<T, NV> ke{c->kbin.cells.as<uint32_t>(), c->pk.as<T>(), kslots(c),
-                     FastDiv((uint32_t)c->kshape[2]), FastDiv((uint32_t)c->kshape[1]),
-                     c->kx.as<double>(), c->ky.as<double>(), c->kz.as<double>(),
-                     c->wx.as<double>(), c->wy.as<double>(), c->wz.as<double>(), {}, 0};
-    for (int q = 0; q < NV; ++q) ke.coef[q] = c->kcoef[q];
-    run_hist_seg<NV>(ke, c->kbin, c->hs.part_seg, c->k_hist.as<double>(), c->k_counts.as<int>(),
-                     0, 0, NV, s);
+    if (c->khalf_ok) {
+      run_hist_khalf<T, NV>(c, c->pk.as<T>(), 0, c->hs.part_seg, c->k_hist.as<double>(),
+                            c->k_counts.as<int>(), 0, 0, NV, s);
+    } else {
+      KEntry<T, NV> ke{c->kbin.cells.as<uint32_t>(), c->pk.as<T>(), kslots(c),
+                       FastDiv((uint32_t)c->kshape[2]), FastDiv((uint32_t)c->kshape[1]),
+                       c->kx.as<double>(), c->ky.as<double>(), c->kz.as<double>(),
+                       c->wx.as<double>(), c->wy.as<double>(), c->wz.as<double>(), {}, 0};
+      for (int q = 0; q < NV; ++q) ke.coef[q] = c->kcoef[q];
+      run_hist_seg<NV>(ke, c->kbin, c->hs.part_seg, c->k_hist.as<double>(),
+                       c->k_counts.as<int>(), 0, 0, NV, s);
+    }
   }
   Stage st(c, FC_STAGE_HIST_R);
   WEntry<TI, NV> we{c->xisub.as<TI>(), c->rbin.cells.as<uint32_t>(), c->rw.as<double>(),
@@ -960,11 +1061,11 @@ T* full_inverse(fc_corr* c, cplx<T>* w) {
   hipStream_t s = c->ctx->stream;
   {
     Stage st(c, FC_STAGE_FFT_X_INV);
-    launch_cols<T, +1>(p.n[0], x_view<T>(p, w), p.tile[0], p.tw0.as<cplx<T>>(), s);
+    launch_cols<T, +1>(p.n[0], x_view<T>(p, w), p.tile[0], p.tw0.as<cplx<T>>(), s, p.var[0]);
   }
   {
     Stage st(c, FC_STAGE_FFT_Y_INV);
-    launch_cols<T, +1>(p.n[1], y_view<T>(p, w), p.tile[1], p.tw1.as<cplx<T>>(), s);
+    launch_cols<T, +1>(p.n[1], y_view<T>(p, w), p.tile[1], p.tw1.as<cplx<T>>(), s, p.var[1]);
   }
   Stage st(c, FC_STAGE_FFT_Z_INV);
   T* xr = reinterpret_cast<T*>(w);
@@ -1267,7 +1368,7 @@ int fc_corr_footprint(fc_corr* c, int interlaced, size_t* held, size_t* full) {
                                       &c->r_counts, &c->r_hist, &c->k_counts, &c->k_hist,
                                       &c->zerolag, &c->hs.part_hist, &c->hs.part_counts})
       add(*b, 0);
-    for (const fc::BinnedCells* bc : {&c->kbin, &c->rbin}) {
+    for (const fc::BinnedCells* bc : {&c->kbin, &c->rbin, &c->khalf}) {
       add(bc->cells, 0);
       add(bc->counts, 0);
       add(bc->chunks, 0);
@@ -1291,7 +1392,8 @@ int fc_corr_footprint(fc_corr* c, int interlaced, size_t* held, size_t* full) {
     add(c->rsub, c->kind == FC_YAMAMOTO || c->algo == FC_ALGO_PLAIN ? 8 * rs * rs * rs : 0);
     add(c->ksub, c->algo == FC_ALGO_PLAIN ? 8 * K : 0);
     add(c->rw, 8 * nv * (size_t)std::max<int64_t>(c->rbin.n, 1));
-    add(c->hs.part_seg, 8 * nv * (size_t)std::max(c->kbin.nchunks, c->rbin.nchunks));
+    add(c->hs.part_seg,
+        8 * nv * (size_t)std::max({c->kbin.nchunks, c->rbin.nchunks, c->khalf.nchunks}));
     for (int d = 0; d < 3; ++d)
       add(c->ph[d], sizeof(double2) * (size_t)(d == 2 ? p.ldc : c->shape[d]));
     if (held) *held = now;
@@ -1512,7 +1614,7 @@ void batch_sequence(fc_batch* bt, int n) {
     ColView<T> v = y_view<T>(p, w);
     v.A = n * p.n[0];
     launch_cols_wide<T, -1, TI>(p.n[1], v, p.tile[1], pi.tw1.as<cplx<TI>>(),
-                                p.tw1.as<cplx<T>>(), s);
+                                p.tw1.as<cplx<T>>(), s, p.var[1]);
   }
   if (px_fits(c)) {
     PxOut po{bt->pk.as<void>(), bt->q1.as<void>(), n, (int64_t)p.n[0] * p.n[1] * p.ldc, bt->npk,
@@ -1528,7 +1630,7 @@ void batch_sequence(fc_batch* bt, int n) {
     for (int i = 0; i < n; ++i) {
       cplx<T>* wi = w + (int64_t)i * nc;
       launch_cols_wide<T, -1, TI>(p.n[0], x_view<T>(p, wi), p.tile[0], pi.tw0.as<cplx<TI>>(),
-                                  p.tw0.as<cplx<T>>(), s);
+                                  p.tw0.as<cplx<T>>(), s, p.var[0]);
       hipLaunchKernelGGL(k_conj_mult<T>, dim3(grid1d(nc, 256, 65536)), dim3(256), 0, s, wi,
                          nullptr, nc);
       FC_LAUNCH_CHECK();
@@ -1538,23 +1640,28 @@ void batch_sequence(fc_batch* bt, int n) {
       ColView<TI, T> v{wi, 0, (int64_t)p.n[1] * p.ldc, (int64_t)p.ldc,
                        bt->q1.as<cplx<TI>>() + i * bt->nq1, 0, (int64_t)p.n[1] * ldq, ldq, 1,
                        p.n[1], p.n2h, rs, o};
-      launch_cols<TI, +1, T>(p.n[0], v, pi.tile[0], pi.tw0.as<cplx<TI>>(), s);
+      launch_cols<TI, +1, T>(p.n[0], v, pi.tile[0], pi.tw0.as<cplx<TI>>(), s, pi.var[0]);
     }
   }
   {
     ColView<TI> v{bt->q1.as<cplx<TI>>(), (int64_t)pi.n[1] * ldq, ldq, 0, bt->q2.as<cplx<TI>>(),
                   (int64_t)rs * ldq, ldq, 0, n * rs, 1, pi.n2h, rs, o};
-    launch_cols<TI, +1>(pi.n[1], v, pi.tile[1], pi.tw1.as<cplx<TI>>(), s);
+    launch_cols<TI, +1>(pi.n[1], v, pi.tile[1], pi.tw1.as<cplx<TI>>(), s, pi.var[1]);
   }
   launch_rows_c2r<TI, TI>(pi, bt->q2.as<cplx<TI>>(), ldq, bt->xisub.as<TI>(), rs, s,
                           (int64_t)n * rs * rs, rs, o);
-  KEntry<T, NV> ke{c->kbin.cells.as<uint32_t>(), bt->pk.as<T>(), kslots(c),
-                   FastDiv((uint32_t)c->kshape[2]), FastDiv((uint32_t)c->kshape[1]),
-                   c->kx.as<double>(), c->ky.as<double>(), c->kz.as<double>(),
-                   c->wx.as<double>(), c->wy.as<double>(), c->wz.as<double>(), {}, bt->npk};
-  for (int q = 0; q < NV; ++q) ke.coef[q] = c->kcoef[q];
-  run_hist_seg<NV>(ke, c->kbin, bt->part, bt->k_hist.as<double>(), bt->k_counts.as<int>(), 0, 0,
-                   NV, s, -1, nullptr, n, hk);
+  if (c->khalf_ok) {
+    run_hist_khalf<T, NV>(c, bt->pk.as<T>(), bt->npk, bt->part, bt->k_hist.as<double>(),
+                          bt->k_counts.as<int>(), 0, 0, NV, s, n, hk);
+  } else {
+    KEntry<T, NV> ke{c->kbin.cells.as<uint32_t>(), bt->pk.as<T>(), kslots(c),
+                     FastDiv((uint32_t)c->kshape[2]), FastDiv((uint32_t)c->kshape[1]),
+                     c->kx.as<double>(), c->ky.as<double>(), c->kz.as<double>(),
+                     c->wx.as<double>(), c->wy.as<double>(), c->wz.as<double>(), {}, bt->npk};
+    for (int q = 0; q < NV; ++q) ke.coef[q] = c->kcoef[q];
+    run_hist_seg<NV>(ke, c->kbin, bt->part, bt->k_hist.as<double>(), bt->k_counts.as<int>(), 0,
+                     0, NV, s, -1, nullptr, n, hk);
+  }
   WEntry<TI, NV> we{bt->xisub.as<TI>(), c->rbin.cells.as<uint32_t>(), c->rw.as<double>(),
                     c->rbin.n, bt->nxs};
   run_hist_seg<NV>(we, c->rbin, bt->part, bt->r_hist.as<double>(), bt->r_counts.as<int>(), 0, 0,
@@ -1633,7 +1740,8 @@ int fc_batch_create(fc_ctx* ctx, const int shape[3], double cell_size, int windo
     b->r_hist.allocate(sizeof(double) * hr * nbatch);
     b->r_counts.allocate(sizeof(int) * hr * nbatch);
     b->zerolag.allocate(sizeof(double) * nbatch);
-    b->part.allocate(sizeof(double) * 3 * (size_t)std::max(c->kbin.nchunks, c->rbin.nchunks) *
+    b->part.allocate(sizeof(double) * 3 *
+                     (size_t)std::max({c->kbin.nchunks, c->rbin.nchunks, c->khalf.nchunks}) *
                      nbatch);
     switch (c->nell) {
       case 1:

@@ -24,6 +24,50 @@ bool fft_len_fast(int L) {
 int fft_len_e(int L) { return (L % 3 == 0) ? 24 : 8; }
 
 namespace {
+template <int... Rs>
+int copy_radices(Radices<Rs...>, int* out) {
+  const int r[] = {Rs...};
+  for (int i = 0; i < (int)sizeof...(Rs); ++i) out[i] = r[i];
+  return (int)sizeof...(Rs);
+}
+template <int L>
+int radices_of(int var, int* out) {
+  if (var == 0) return copy_radices(typename FftCfgV<L, 0>::radices{}, out);
+  if constexpr (FftCfgV<L, 1>::fast)
+    if (var == 1) return copy_radices(typename FftCfgV<L, 1>::radices{}, out);
+  return 0;
+}
+}  // namespace
+
+int fft_radices(int L, int var, int* out) {
+  switch (L) {
+    case 8: return radices_of<8>(var, out);
+    case 16: return radices_of<16>(var, out);
+    case 32: return radices_of<32>(var, out);
+    case 64: return radices_of<64>(var, out);
+    case 128: return radices_of<128>(var, out);
+    case 256: return radices_of<256>(var, out);
+    case 512: return radices_of<512>(var, out);
+    default: return 0;
+  }
+}
+
+bool fft_var_ok(int L, int var) {
+  if (var == 0) return true;
+  if (var != 1 || !rc_len_ok(L)) return false;
+  switch (L) {
+    case 8: return FftCfgV<8, 1>::fast;
+    case 16: return FftCfgV<16, 1>::fast;
+    case 32: return FftCfgV<32, 1>::fast;
+    case 64: return FftCfgV<64, 1>::fast;
+    case 128: return FftCfgV<128, 1>::fast;
+    case 256: return FftCfgV<256, 1>::fast;
+    case 512: return FftCfgV<512, 1>::fast;
+    default: return false;
+  }
+}
+
+namespace {
 
 struct PlanKey {
   int device, n0, n1, n2, prec;
@@ -36,49 +80,48 @@ std::mutex g_plan_mu;
 std::map<PlanKey, std::shared_ptr<FftPlan>> g_plans;
 // Tile overrides (imported wisdom or the autotuner): (axis length, precision,
 // kind) -> tile.  kind 0 = column pass (columns per workgroup), 1 = row pass
-// (rows per workgroup), 2 = z+y chunk size in KiB (length 0).
+// (rows per workgroup), 2 = z+y chunk size in KiB (length 0), 3 = column
+// pass plan variant, 4 = row pass plan variant.
 std::map<std::tuple<int, int, int>, int> g_tile_wisdom;
 
-int stage_len(int L) {
+// Stage tables of every plan variant of length L, in variant order (see
+// stage_twiddle_base): put(index after the plain part, plain index m).
+template <int L, class Put>
+int stage_entries(Put put) {
+  stage_twiddles_<L>(typename FftCfgV<L, 0>::radices{}, put);
+  const int b1 = stage_twiddle_base<L, 1>();
+  if constexpr (FftCfgV<L, 1>::fast)
+    stage_twiddles_<L>(typename FftCfgV<L, 1>::radices{},
+                       [&](int idx, int m) { put(b1 + idx, m); });
+  return stage_twiddle_base<L, kFftVariants>();
+}
+
+template <class Put>
+int stage_entries_for(int L, Put put) {
   switch (L) {
-    case 8: return stage_twiddle_len<8>();
-    case 16: return stage_twiddle_len<16>();
-    case 32: return stage_twiddle_len<32>();
-    case 64: return stage_twiddle_len<64>();
-    case 128: return stage_twiddle_len<128>();
-    case 256: return stage_twiddle_len<256>();
-    case 512: return stage_twiddle_len<512>();
+    case 8: return stage_entries<8>(put);
+    case 16: return stage_entries<16>(put);
+    case 32: return stage_entries<32>(put);
+    case 64: return stage_entries<64>(put);
+    case 128: return stage_entries<128>(put);
+    case 256: return stage_entries<256>(put);
+    case 512: return stage_entries<512>(put);
     default: return 0;
   }
 }
 
-template <int L, class Put>
-void stage_entries(Put put) {
-  stage_twiddles_<L>(typename FftCfg<L>::radices{}, put);
-}
-
 // Plain table exp(-2 pi i m / L), m < L, followed (register-chained lengths)
-// by the per-stage table of the same values regrouped per thread.
+// by the per-stage tables of the same values regrouped per thread.
 template <typename T>
 void upload_twiddles(DeviceBuffer& buf, int L) {
-  const int ns = rc_len_ok(L) ? stage_len(L) : 0;
+  const int ns = rc_len_ok(L) ? stage_entries_for(L, [](int, int) {}) : 0;
   std::vector<cplx<T>> h(L + ns);
   for (int m = 0; m < L; ++m) {
     long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)m / L;
     h[m].x = (T)cosl(a);
     h[m].y = (T)(-sinl(a));
   }
-  auto put = [&](int idx, int m) { h[L + idx] = h[m]; };
-  switch (ns ? L : 0) {
-    case 8: stage_entries<8>(put); break;
-    case 16: stage_entries<16>(put); break;
-    case 32: stage_entries<32>(put); break;
-    case 64: stage_entries<64>(put); break;
-    case 128: stage_entries<128>(put); break;
-    case 256: stage_entries<256>(put); break;
-    case 512: stage_entries<512>(put); break;
-    default: break;
-  }
+  if (ns) stage_entries_for(L, [&](int idx, int m) { h[L + idx] = h[m]; });
   buf.allocate(sizeof(cplx<T>) * h.size());
   FC_HIP(hipMemcpy(buf.as<void>(), h.data(), sizeof(cplx<T>) * h.size(), hipMemcpyHostToDevice));
 }
@@ -131,6 +174,14 @@ std::shared_ptr<FftPlan> build_plan(int device, const int shape[3], int prec) {
   p->tile[0] = tile_for(shape[0], 0, default_col_tile(shape[0], cs));
   p->tile[1] = tile_for(shape[1], 0, default_col_tile(shape[1], cs));
   p->tile[2] = p->fast[2] ? tile_for(shape[2] / 2, 1, default_row_tile(shape[2] / 2, cs)) : 1;
+  for (int ax = 0; ax < 2; ++ax) {
+    const int v = tile_for(shape[ax], 3, 0);
+    p->var[ax] = fft_var_ok(shape[ax], v) ? v : 0;
+  }
+  if (p->fast[2]) {
+    const int v = tile_for(shape[2] / 2, 4, 0);
+    p->var[2] = fft_var_ok(shape[2] / 2, v) ? v : 0;
+  }
   p->zy_chunk_kib = tile_for(0, 2, kDefaultZyChunkKiB);
   {
     const size_t slab = (size_t)shape[1] * p->ldc * cs;
@@ -283,9 +334,14 @@ int fc_plan_cache_export(const char* path) {
     std::map<std::tuple<int, int, int>, int> all = fc::g_tile_wisdom;
     for (auto& kv : fc::g_plans) {
       const auto& p = *kv.second;
-      for (int ax = 0; ax < 2; ++ax)
+      for (int ax = 0; ax < 2; ++ax) {
         all.emplace(std::make_tuple(p.n[ax], p.prec, 0), p.tile[ax]);
-      if (p.fast[2]) all.emplace(std::make_tuple(p.n[2] / 2, p.prec, 1), p.tile[2]);
+        all.emplace(std::make_tuple(p.n[ax], p.prec, 3), p.var[ax]);
+      }
+      if (p.fast[2]) {
+        all.emplace(std::make_tuple(p.n[2] / 2, p.prec, 1), p.tile[2]);
+        all.emplace(std::make_tuple(p.n[2] / 2, p.prec, 4), p.var[2]);
+      }
       all.emplace(std::make_tuple(0, 0, 2), p.zy_chunk_kib);
     }
     for (auto& kv : all)
@@ -307,13 +363,22 @@ int fc_plan_cache_import(const char* path) {
     int L, prec, kind, tile;
     std::vector<std::array<int, 4>> rows;
     while (f >> L >> prec >> kind >> tile) {
-      const bool ok = kind == 2 ? (L == 0 && prec == 0 && tile >= 0)
-                                : (L > 0 && tile > 0 && (prec == FC_FP64 || prec == FC_FP32) &&
-                                   (kind == 0 || kind == 1));
+      const bool prec_ok = prec == FC_FP64 || prec == FC_FP32;
+      const bool ok = kind == 2   ? (L == 0 && prec == 0 && tile >= 0)
+                      : kind >= 3 ? (kind <= 4 && L > 0 && prec_ok && tile >= 0 &&
+                                     tile < fc::kFftVariants)
+                                  : ((kind == 0 || kind == 1) && L > 0 && tile > 0 && prec_ok);
       if (!ok) fc::fail(FC_ERR_RUNTIME, std::string("corrupt plan cache file: ") + path);
       rows.push_back({L, prec, kind, tile});
     }
     for (auto& r : rows) fc::set_tile_wisdom(r[0], r[1], r[2], r[3]);
+  });
+}
+
+int fc_fft_radices(int L, int var, int radices[8], int* count) {
+  return fc::guard([&] {
+    if (!radices || !count) fc::fail(FC_ERR_INVALID, "null argument");
+    *count = fc::fft_radices(L, var, radices);
   });
 }
 

@@ -39,15 +39,15 @@ template <typename T>
 using cplx = typename CplxOf<T>::type;
 
 template <typename C>
-__device__ __forceinline__ C cadd(C a, C b) {
+__host__ __device__ __forceinline__ C cadd(C a, C b) {
   return C{a.x + b.x, a.y + b.y};
 }
 template <typename C>
-__device__ __forceinline__ C csub(C a, C b) {
+__host__ __device__ __forceinline__ C csub(C a, C b) {
   return C{a.x - b.x, a.y - b.y};
 }
 template <typename C>
-__device__ __forceinline__ C cmul(C a, C b) {
+__host__ __device__ __forceinline__ C cmul(C a, C b) {
   return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 // Complex value in another precision (the storage type of a pass whose
@@ -97,7 +97,7 @@ struct ScalarOf<double2> {
 };
 // a * (DIR * i): DIR = -1 multiplies by -i, DIR = +1 by +i.
 template <int DIR, typename C>
-__device__ __forceinline__ C mul_i(C a) {
+__host__ __device__ __forceinline__ C mul_i(C a) {
   return DIR < 0 ? C{a.y, -a.x} : C{-a.y, a.x};
 }
 
@@ -108,7 +108,7 @@ struct Bfly;
 template <int DIR, typename T>
 struct Bfly<2, DIR, T> {
   using C = cplx<T>;
-  static __device__ __forceinline__ void run(C* v) {
+  static __host__ __device__ __forceinline__ void run(C* v) {
     C a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
@@ -118,7 +118,7 @@ struct Bfly<2, DIR, T> {
 template <int DIR, typename T>
 struct Bfly<4, DIR, T> {
   using C = cplx<T>;
-  static __device__ __forceinline__ void run(C* v) {
+  static __host__ __device__ __forceinline__ void run(C* v) {
     C t0 = cadd(v[0], v[2]);
     C t1 = csub(v[0], v[2]);
     C t2 = cadd(v[1], v[3]);
@@ -142,20 +142,22 @@ template <>
 struct ConstMul<double> {
   double c;
   __host__ __device__ constexpr ConstMul(long double v) : c((double)v) {}
-  __device__ __forceinline__ double operator()(double x) const { return c * x; }
+  __host__ __device__ __forceinline__ double operator()(double x) const { return c * x; }
 };
 template <>
 struct ConstMul<float> {
   float hi, lo;
   __host__ __device__ constexpr ConstMul(long double v)
       : hi((float)v), lo((float)(v - (long double)(float)v)) {}
-  __device__ __forceinline__ float operator()(float x) const { return __builtin_fmaf(hi, x, lo * x); }
+  __host__ __device__ __forceinline__ float operator()(float x) const {
+    return __builtin_fmaf(hi, x, lo * x);
+  }
 };
 
 template <int DIR, typename T>
 struct Bfly<8, DIR, T> {
   using C = cplx<T>;
-  static __device__ __forceinline__ void run(C* v) {
+  static __host__ __device__ __forceinline__ void run(C* v) {
     C e[4] = {v[0], v[2], v[4], v[6]};
     C o[4] = {v[1], v[3], v[5], v[7]};
     Bfly<4, DIR, T>::run(e);
@@ -176,10 +178,56 @@ struct Bfly<8, DIR, T> {
   }
 };
 
+// Radix 16 as 4 x 4: with n = 4 n1 + n2 and k = k1 + 4 k2,
+// X[k1 + 4 k2] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 W4^(n1 k1) x[4 n1 + n2]:
+// four radix-4 butterflies over n1, the inner twiddles W16^(n2 k1), then four
+// radix-4 butterflies over n2.
+template <int DIR, typename T>
+struct Bfly<16, DIR, T> {
+  using C = cplx<T>;
+  // x * (a + DIR i b) for the constant a + DIR i b.
+  static __host__ __device__ __forceinline__ C rot(C x, ConstMul<T> a, ConstMul<T> b) {
+    return C{a(x.x) - DIR * b(x.y), a(x.y) + DIR * b(x.x)};
+  }
+  static __host__ __device__ __forceinline__ void run(C* v) {
+    constexpr ConstMul<T> h(0.70710678118654752440084436210484903928L);
+    constexpr ConstMul<T> c1(0.92387953251128675612818318939678828682L);  // cos(pi/8)
+    constexpr ConstMul<T> s1(0.38268343236508977172845998403039886676L);  // sin(pi/8)
+    C y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      C a[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+      Bfly<4, DIR, T>::run(a);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = a[k1];
+    }
+    // W16^1, ^2, ^3 on row 1; ^2, ^4, ^6 on row 2; ^3, ^6, ^9 on row 3.
+    y[1][1] = rot(y[1][1], c1, s1);
+    y[1][2] = C{h(y[1][2].x - DIR * y[1][2].y), h(y[1][2].y + DIR * y[1][2].x)};
+    y[1][3] = rot(y[1][3], s1, c1);
+    y[2][1] = C{h(y[2][1].x - DIR * y[2][1].y), h(y[2][1].y + DIR * y[2][1].x)};
+    y[2][2] = mul_i<DIR>(y[2][2]);
+    y[2][3] = C{h(-y[2][3].x - DIR * y[2][3].y), h(-y[2][3].y + DIR * y[2][3].x)};
+    y[3][1] = rot(y[3][1], s1, c1);
+    y[3][2] = C{h(-y[3][2].x - DIR * y[3][2].y), h(-y[3][2].y + DIR * y[3][2].x)};
+    {
+      const C r = rot(y[3][3], c1, s1);  // W16^9 = -W16^1
+      y[3][3] = C{-r.x, -r.y};
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      C b[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+      Bfly<4, DIR, T>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = b[k2];
+    }
+  }
+};
+
 template <int DIR, typename T>
 struct Bfly<3, DIR, T> {
   using C = cplx<T>;
-  static __device__ __forceinline__ void run(C* v) {
+  static __host__ __device__ __forceinline__ void run(C* v) {
     constexpr ConstMul<T> s3(0.86602540378443864676372317075293618347L);  // sqrt(3)/2
     C t = cadd(v[1], v[2]);
     C d = mul_i<DIR>(csub(v[1], v[2]));
@@ -226,6 +274,29 @@ FC_FFT_CFG(768, 24, 8, 8, 4, 3)
 FC_FFT_CFG(1536, 24, 8, 8, 8, 3)
 FC_FFT_CFG(3072, 24, 8, 8, 8, 2, 3)
 #undef FC_FFT_CFG
+
+// Plan variants of the register-chained kernels, chosen per pass by the
+// autotuner: 0 is the table above (first radix 8); 1 starts with radix 16
+// (16 values per thread, half the threads per column or row, and for
+// 256 = 16 x 16 one LDS exchange instead of two).
+constexpr int kFftVariants = 2;
+template <int L, int V>
+struct FftCfgV : FftCfg<L> {};
+template <int L>
+struct FftCfgV<L, 1> {
+  static constexpr bool fast = false;
+};
+#define FC_FFT_CFG16(Lv, ...)               \
+  template <>                               \
+  struct FftCfgV<Lv, 1> {                   \
+    static constexpr bool fast = true;      \
+    static constexpr int E = 16;            \
+    using radices = Radices<__VA_ARGS__>;   \
+  };
+FC_FFT_CFG16(128, 16, 8)
+FC_FFT_CFG16(256, 16, 16)
+FC_FFT_CFG16(512, 16, 16, 2)
+#undef FC_FFT_CFG16
 
 // LDS addressing: element e of column c.
 struct LayColFast {  // [e][TC]: columns contiguous (strided y/x passes)
@@ -313,21 +384,22 @@ __host__ __device__ constexpr bool rc_len_ok(int L) {
   return L >= 8 && L <= 512 && (L & (L - 1)) == 0;
 }
 
-template <int L>
+template <int L, int V = 0>
 __host__ __device__ constexpr int out_m(int s) {
-  constexpr int E = FftCfg<L>::E;
-  constexpr int Rl = RadixInfo<typename FftCfg<L>::radices>::last;
+  constexpr int E = FftCfgV<L, V>::E;
+  constexpr int Rl = RadixInfo<typename FftCfgV<L, V>::radices>::last;
   return s / Rl + (s % Rl) * (E / Rl);
 }
 
 // Column tiles: TC columns of length L, element (e, c) at row
-// e ^ ((e >> 3) & 1): rows 8a..8a+7 swap neighbours when bit 3 of the row is
-// set, so the first-stage stores of threads t and t + 1 (rows 8t + r) fall in
-// different LDS banks when a row is narrower than the 128 bytes of banks.
-template <int TC>
+// e ^ ((e / E) & 1) (E = values per thread, a power of two): rows Ea..Ea+E-1
+// swap neighbours when bit log2(E) of the row is set, so the first-stage
+// stores of threads t and t + 1 (rows Et + r) fall in different LDS banks
+// when a row is narrower than the 128 bytes of banks.
+template <int TC, int E = 8>
 struct LaySwz {
   __device__ __forceinline__ int operator()(int c, int e) const {
-    return (e ^ ((e >> 3) & 1)) * TC + c;
+    return (e ^ ((e / E) & 1)) * TC + c;
   }
 };
 // Row tiles: row c of rs slots, element e at e + e / 8 (one pad slot per 8),
@@ -352,12 +424,12 @@ __host__ __device__ constexpr int row_pad_len(int M) { return M + M / 8 + 1; }
 // entries at OFF, entry k * R + r = exp(-2 pi i r k / (NS R)), so a thread's
 // R - 1 twiddles of one stage are contiguous.
 template <typename T, int L, int R, int NS, int OFF, int DIR, int NT, bool FIRST, bool LAST,
-          class Lay, int Q0 = 0, int NQ = NT, class CL = cplx<T>>
-__device__ __forceinline__ void rstage(cplx<T> (&v)[NT][FftCfg<L>::E], CL* lds, int ts,
+          class Lay, int Q0, int NQ, class CL, int V>
+__device__ __forceinline__ void rstage(cplx<T> (&v)[NT][(FftCfgV<L, V>::E)], CL* lds, int ts,
                                        const cplx<T>* __restrict__ tws, int c, int t, Lay lay) {
   using TL = typename ScalarOf<CL>::type;
   using C = cplx<T>;
-  constexpr int E = FftCfg<L>::E;
+  constexpr int E = FftCfgV<L, V>::E;
   constexpr int TCOL = L / E;
   constexpr int NB = E / R;
   constexpr int LR = L / R;
@@ -415,28 +487,20 @@ __device__ __forceinline__ void rstage(cplx<T> (&v)[NT][FftCfg<L>::E], CL* lds, 
 }
 
 template <typename T, int L, int DIR, int NT, int NS, int OFF, bool FIRST, int Q0, int NQ,
-          class Lay, class CL>
-__device__ __forceinline__ void rstages_(Radices<>, cplx<T> (&)[NT][FftCfg<L>::E], CL*, int,
-                                         const cplx<T>*, int, int, Lay) {}
+          class Lay, class CL, int V>
+__device__ __forceinline__ void rstages_(Radices<>, cplx<T> (&)[NT][(FftCfgV<L, V>::E)], CL*,
+                                         int, const cplx<T>*, int, int, Lay) {}
 
 template <typename T, int L, int DIR, int NT, int NS, int OFF, bool FIRST, int Q0, int NQ,
-          class Lay, class CL, int R, int... Rest>
-__device__ __forceinline__ void rstages_(Radices<R, Rest...>, cplx<T> (&v)[NT][FftCfg<L>::E],
-                                         CL* lds, int ts, const cplx<T>* __restrict__ tws,
-                                         int c, int t, Lay lay) {
-  rstage<T, L, R, NS, OFF, DIR, NT, FIRST, sizeof...(Rest) == 0, Lay, Q0, NQ, CL>(v, lds, ts,
-                                                                                   tws, c, t, lay);
-  rstages_<T, L, DIR, NT, NS * R, (NS > 1 ? OFF + NS * R : OFF), false, Q0, NQ, Lay, CL>(
+          class Lay, class CL, int V, int R, int... Rest>
+__device__ __forceinline__ void rstages_(Radices<R, Rest...>,
+                                         cplx<T> (&v)[NT][(FftCfgV<L, V>::E)], CL* lds, int ts,
+                                         const cplx<T>* __restrict__ tws, int c, int t,
+                                         Lay lay) {
+  rstage<T, L, R, NS, OFF, DIR, NT, FIRST, sizeof...(Rest) == 0, Lay, Q0, NQ, CL, V>(
+      v, lds, ts, tws, c, t, lay);
+  rstages_<T, L, DIR, NT, NS * R, (NS > 1 ? OFF + NS * R : OFF), false, Q0, NQ, Lay, CL, V>(
       Radices<Rest...>{}, v, lds, ts, tws, c, t, lay);
-}
-
-// All stages; tw is the axis's twiddle table (plain part then stage part).
-// Tiles Q0 .. Q0 + NQ - 1 of v run in lock-step (LDS tile q - Q0).
-template <typename T, int L, int DIR, int NT, int NS, bool FIRST, class Lay, class Rad,
-          int Q0 = 0, int NQ = NT, class CL>
-__device__ __forceinline__ void rstages(Rad, cplx<T> (&v)[NT][FftCfg<L>::E], CL* lds, int ts,
-                                        const cplx<T>* __restrict__ tw, int c, int t, Lay lay) {
-  rstages_<T, L, DIR, NT, NS, 0, FIRST, Q0, NQ, Lay, CL>(Rad{}, v, lds, ts, tw + L, c, t, lay);
 }
 
 // Host: length of the per-stage table of an L-point register-chained plan,
@@ -448,9 +512,33 @@ constexpr int stage_twiddle_len_(Radices<R, Rest...>) {
   else
     return (NS > 1 ? NS * R : 0) + stage_twiddle_len_<L, NS * R>(Radices<Rest...>{});
 }
-template <int L>
+template <int L, int V = 0>
 constexpr int stage_twiddle_len() {
-  return stage_twiddle_len_<L>(typename FftCfg<L>::radices{});
+  if constexpr (FftCfgV<L, V>::fast)
+    return stage_twiddle_len_<L>(typename FftCfgV<L, V>::radices{});
+  else
+    return 0;
+}
+// An axis table holds the L plain twiddles, then the stage tables of
+// variant 0, 1, ... in turn; variant V's starts stage_twiddle_base<L, V>()
+// entries after the plain part.
+template <int L, int V>
+constexpr int stage_twiddle_base() {
+  if constexpr (V == 0)
+    return 0;
+  else
+    return stage_twiddle_base<L, V - 1>() + stage_twiddle_len<L, V - 1>();
+}
+
+// All stages of plan variant V; tw is the axis's twiddle table.  Tiles
+// Q0 .. Q0 + NQ - 1 of v run in lock-step (LDS tile q - Q0).
+template <typename T, int L, int DIR, int NT, int NS, bool FIRST, int V, class Lay, int Q0 = 0,
+          int NQ = NT, class CL>
+__device__ __forceinline__ void rstages(cplx<T> (&v)[NT][(FftCfgV<L, V>::E)], CL* lds, int ts,
+                                        const cplx<T>* __restrict__ tw, int c, int t, Lay lay) {
+  rstages_<T, L, DIR, NT, NS, 0, FIRST, Q0, NQ, Lay, CL, V>(
+      typename FftCfgV<L, V>::radices{}, v, lds, ts, tw + L + stage_twiddle_base<L, V>(), c, t,
+      lay);
 }
 template <int L, int NS = 1, int OFF = 0, int R, int... Rest, class Put>
 void stage_twiddles_(Radices<R, Rest...>, Put put) {
@@ -477,21 +565,22 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
 // before writing it.  Arithmetic in T; loads Tin; LDS and stores in To.
 // Waves per SIMD a register-chained column kernel is held to when its
 // arithmetic is wider than its LDS tiles (the occupancy of the narrow form).
-template <typename T, typename To>
+// Variant 1 holds twice the values per thread and is sized for half the waves.
+template <typename T, typename To, int V = 0>
 constexpr int rc_wide_waves() {
-  return sizeof(T) > sizeof(To) ? 8 : 1;
+  return sizeof(T) > sizeof(To) ? (V == 1 ? 4 : 8) : 1;
 }
 
-template <typename T, int L, int DIR, int TC, typename Tin = T, typename To = T>
-__global__ void __launch_bounds__(TC*(L / FftCfg<L>::E))
-__attribute__((amdgpu_waves_per_eu(rc_wide_waves<T, To>())))
+template <typename T, int L, int DIR, int TC, typename Tin = T, typename To = T, int V = 0>
+__global__ void __launch_bounds__(TC*(L / FftCfgV<L, V>::E))
+__attribute__((amdgpu_waves_per_eu(rc_wide_waves<T, To, V>())))
     k_fft_cols_rc(const cplx<Tin>* in, int64_t inA, int64_t inL, int64_t inG, cplx<To>* out,
                   int64_t outA, int64_t outL, int64_t outG, int B, int tilesB, int G, int nout,
                   int offm, const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
-  constexpr int E = FftCfg<L>::E;
+  constexpr int E = FftCfgV<L, V>::E;
   constexpr int TCOL = L / E;
-  using Rad = typename FftCfg<L>::radices;
+  using Rad = typename FftCfgV<L, V>::radices;
   static_assert(RadixInfo<Rad>::first == E, "first radix must equal E");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<To>* lds = reinterpret_cast<cplx<To>*>(smem_raw);
@@ -521,11 +610,11 @@ __attribute__((amdgpu_waves_per_eu(rc_wide_waves<T, To>())))
       v[0][s] = C{T(0), T(0)};
     }
   }
-  rstages<T, L, DIR, 1, 1, true>(Rad{}, v, lds, 0, tw, c, t, LaySwz<TC>{});
+  rstages<T, L, DIR, 1, 1, true, V>(v, lds, 0, tw, c, t, LaySwz<TC, E>{});
   if (!colok) return;
 #pragma unroll
   for (int s = 0; s < E; ++s) {
-    int o = t + out_m<L>(s) * TCOL + offm;
+    int o = t + out_m<L, V>(s) * TCOL + offm;
     if (o >= L) o -= L;
     for (; o < nout; o += L)
       st_stream(reinterpret_cast<cplx<To>*>(dstb + (cout + (uint32_t)o * outLb)),
@@ -560,16 +649,16 @@ struct PairOf<double> {
 // values, a length-M FFT, then the Hermitian post-process (see
 // k_fft_rows_r2c).  Block = tr rows x M/E threads, LDS tr row_pad_len(M).
 // Arithmetic in T; LDS and outputs in To.
-template <typename T, typename Tin, int M, class Loader, typename To = T>
+template <typename T, typename Tin, int M, class Loader, typename To = T, int V = 0>
 __global__ void __launch_bounds__(1024)
     k_fft_rows_r2c_rc(const Tin* in, int64_t ldi, cplx<To>* out, int64_t ldo, int64_t nrows,
                       int tr, const cplx<T>* __restrict__ twM, const cplx<T>* __restrict__ twN,
                       Loader ld) {
   using C = cplx<T>;
-  constexpr int E = FftCfg<M>::E;
+  constexpr int E = FftCfgV<M, V>::E;
   constexpr int TCOL = M / E;
   constexpr int RS = row_pad_len(M);
-  using Rad = typename FftCfg<M>::radices;
+  using Rad = typename FftCfgV<M, V>::radices;
   static_assert(RadixInfo<Rad>::first == E, "first radix must equal E");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<To>* lds = reinterpret_cast<cplx<To>*>(smem_raw);
@@ -593,10 +682,10 @@ __global__ void __launch_bounds__(1024)
       v[0][s] = z;
     }
   }
-  rstages<T, M, -1, 1, 1, true>(Rad{}, v, lds, 0, twM, c, t, lay);
+  rstages<T, M, -1, 1, 1, true, V>(v, lds, 0, twM, c, t, lay);
   __syncthreads();  // the last stage's LDS reads precede the stores below
 #pragma unroll
-  for (int s = 0; s < E; ++s) lds[lay(c, t + out_m<M>(s) * TCOL)] = to_cplx<To>(v[0][s]);
+  for (int s = 0; s < E; ++s) lds[lay(c, t + out_m<M, V>(s) * TCOL)] = to_cplx<To>(v[0][s]);
   __syncthreads();
   // Hermitian half: X[k] = (Z[k] + conj Z[M-k])/2 - i W^k (Z[k] - conj Z[M-k])/2
   if (!rowok) return;
@@ -624,16 +713,16 @@ __global__ void __launch_bounds__(1024)
 // writes sample (o - off) mod 2M at o < nsel (offm = off mod 2M).  In place
 // when the real rows overlay the complex rows: every workgroup reads its rows
 // before writing them.
-template <typename T, typename Tout, int M>
+template <typename T, typename Tout, int M, int V = 0>
 __global__ void __launch_bounds__(1024)
     k_fft_rows_c2r_rc(const cplx<T>* in, int64_t ldi, Tout* out, int64_t ldo, int64_t nrows,
                       int tr, const cplx<T>* __restrict__ twM, const cplx<T>* __restrict__ twN,
                       int nsel, int offm) {
   using C = cplx<T>;
-  constexpr int E = FftCfg<M>::E;
+  constexpr int E = FftCfgV<M, V>::E;
   constexpr int TCOL = M / E;
   constexpr int RS = row_pad_len(M);
-  using Rad = typename FftCfg<M>::radices;
+  using Rad = typename FftCfgV<M, V>::radices;
   static_assert(RadixInfo<Rad>::first == E, "first radix must equal E");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
@@ -674,20 +763,20 @@ __global__ void __launch_bounds__(1024)
 #pragma unroll
   for (int s = 0; s < E; ++s) v[0][s] = lds[lay(c, t + s * TCOL)];
   __syncthreads();  // every first-stage input is read before the first stage stores
-  rstages<T, M, +1, 1, 1, true>(Rad{}, v, lds, 0, twM, c, t, lay);
+  rstages<T, M, +1, 1, 1, true, V>(v, lds, 0, twM, c, t, lay);
   if (!rowok) return;
   Tout* dst = out + row * ldo;
   if (nsel == 0) {
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      const int e = t + out_m<M>(s) * TCOL;
+      const int e = t + out_m<M, V>(s) * TCOL;
       dst[2 * e] = Tout(v[0][s].x);
       dst[2 * e + 1] = Tout(v[0][s].y);
     }
   } else {
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      const int e = t + out_m<M>(s) * TCOL;
+      const int e = t + out_m<M, V>(s) * TCOL;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         int o = 2 * e + h + offm;

@@ -5,6 +5,7 @@
 // callers can fuse their own loads into the z pass.
 #pragma once
 
+#include <algorithm>
 #include <type_traits>
 
 #include "fc_internal.h"
@@ -32,6 +33,9 @@ struct FftPlan {
   int prec = FC_FP64;
   bool fast[3] = {false, false, false};  // axis 2: the half-length row FFT
   int tile[3] = {0, 0, 0};
+  // Register-chained plan variant per axis (FftCfgV; axis 2: the row FFT of
+  // length n2/2), 0 unless the autotuner or imported wisdom chose another.
+  int var[3] = {0, 0, 0};
   // x slabs per z+y chunk: the y pass of a chunk reads the z pass's output
   // while it is still in the last-level cache (0: the whole grid at once).
   int zy_chunk = 0;
@@ -46,6 +50,8 @@ struct FftPlan {
 
 bool fft_len_fast(int L);
 int fft_len_e(int L);
+// Whether the register-chained kernels have plan variant var for length L.
+bool fft_var_ok(int L, int var);
 
 // Plan cache keyed by (device, shape, precision).
 std::shared_ptr<FftPlan> get_fft_plan(int device, const int shape[3], int prec);
@@ -67,22 +73,34 @@ struct ColView {
   int nout = 0, off = 0;  // nout == 0: all L outputs
 };
 
-// Register-chained column pass: arithmetic in A (twiddles tw in A), loads
-// Tin, LDS and stores in T.
+template <typename T, int DIR, typename Tin, int TC, typename A, int Lv, int V>
+void launch_cols_rc_v(const ColView<T, Tin>& v, int64_t nb, int tilesB, int nout, int offm,
+                      const cplx<A>* tw, hipStream_t s) {
+  hipLaunchKernelGGL((k_fft_cols_rc<A, Lv, DIR, TC, Tin, T, V>), dim3((unsigned)nb),
+                     dim3(TC * (Lv / FftCfgV<Lv, V>::E)), (size_t)Lv * TC * sizeof(cplx<T>), s,
+                     v.in, v.inA, v.inL, v.inG, v.out, v.outA, v.outL, v.outG, v.B, tilesB, v.G,
+                     nout, offm, tw);
+}
+
+// Register-chained column pass with plan variant var: arithmetic in A
+// (twiddles tw in A), loads Tin, LDS and stores in T.
 template <typename T, int DIR, typename Tin, int TC, typename A = T>
-void launch_cols_rc(int L, const ColView<T, Tin>& v, const cplx<A>* tw, hipStream_t s) {
-  const size_t cs = sizeof(cplx<T>);
+void launch_cols_rc(int L, const ColView<T, Tin>& v, const cplx<A>* tw, hipStream_t s,
+                    int var = 0) {
   const int nout = v.nout ? v.nout : L;
   const int offm = ((v.off % L) + L) % L;
   const int tilesB = (v.B + TC - 1) / TC;
   const int64_t nb = (int64_t)v.A * v.G * tilesB;
   if (nb > INT32_MAX) fail(FC_ERR_INVALID, "transform too large for one launch");
-#define FC_COLS_RC_CASE(Lv)                                                                  \
-  case Lv:                                                                                   \
-    hipLaunchKernelGGL((k_fft_cols_rc<A, Lv, DIR, TC, Tin, T>), dim3((unsigned)nb),          \
-                       dim3(TC * (Lv / FftCfg<Lv>::E)), (size_t)Lv * TC * cs, s, v.in, v.inA, \
-                       v.inL, v.inG, v.out, v.outA, v.outL, v.outG, v.B, tilesB, v.G, nout,  \
-                       offm, tw);                                                            \
+#define FC_COLS_RC_CASE(Lv)                                                              \
+  case Lv:                                                                               \
+    if constexpr (FftCfgV<Lv, 1>::fast) {                                                \
+      if (var == 1) {                                                                    \
+        launch_cols_rc_v<T, DIR, Tin, TC, A, Lv, 1>(v, nb, tilesB, nout, offm, tw, s);   \
+        break;                                                                           \
+      }                                                                                  \
+    }                                                                                    \
+    launch_cols_rc_v<T, DIR, Tin, TC, A, Lv, 0>(v, nb, tilesB, nout, offm, tw, s);       \
     break;
   switch (L) {
     FC_COLS_RC_CASE(8)
@@ -100,14 +118,16 @@ void launch_cols_rc(int L, const ColView<T, Tin>& v, const cplx<A>* tw, hipStrea
 }
 
 template <typename T, int DIR, typename Tin = T>
-void launch_cols(int L, const ColView<T, Tin>& v, int tc, const cplx<T>* tw, hipStream_t s) {
+void launch_cols(int L, const ColView<T, Tin>& v, int tc, const cplx<T>* tw, hipStream_t s,
+                 int var = 0) {
   const size_t cs = sizeof(cplx<T>);
   const int nout = v.nout ? v.nout : L;
   if (rc_len_ok(L) && (size_t)L * tc * cs <= kLdsBudget && tc * (L / 8) <= 1024 &&
       rc_offsets_fit<T, Tin>(L, tc, v.inL, v.outL, nout)) {
-    if (tc == 16) return launch_cols_rc<T, DIR, Tin, 16>(L, v, tw, s);
-    if (tc == 8) return launch_cols_rc<T, DIR, Tin, 8>(L, v, tw, s);
-    if (tc == 4) return launch_cols_rc<T, DIR, Tin, 4>(L, v, tw, s);
+    const int vr = fft_var_ok(L, var) ? var : 0;
+    if (tc == 16) return launch_cols_rc<T, DIR, Tin, 16>(L, v, tw, s, vr);
+    if (tc == 8) return launch_cols_rc<T, DIR, Tin, 8>(L, v, tw, s, vr);
+    if (tc == 4) return launch_cols_rc<T, DIR, Tin, 4>(L, v, tw, s, vr);
   }
 #define FC_COLS_CASE(Lv)                                                                  \
   case Lv: {                                                                              \
@@ -152,18 +172,19 @@ void launch_cols(int L, const ColView<T, Tin>& v, int tc, const cplx<T>* tw, hip
 // register-chained kernel takes L (tw_a: twiddles in A), else in T (tw_t).
 template <typename T, int DIR, typename A>
 void launch_cols_wide(int L, const ColView<T>& v, int tc, const cplx<A>* tw_a,
-                      const cplx<T>* tw_t, hipStream_t s) {
+                      const cplx<T>* tw_t, hipStream_t s, int var = 0) {
   if constexpr (!std::is_same<T, A>::value) {
     int t = tc;
     while (t > 4 && ((size_t)L * t * sizeof(cplx<T>) > kLdsBudget || t * (L / 8) > 1024)) t /= 2;
     if (rc_len_ok(L) && (size_t)L * t * sizeof(cplx<T>) <= kLdsBudget && t * (L / 8) <= 1024 &&
         rc_offsets_fit<T, T>(L, t, v.inL, v.outL, v.nout ? v.nout : L)) {
-      if (t >= 16) return launch_cols_rc<T, DIR, T, 16, A>(L, v, tw_a, s);
-      if (t == 8) return launch_cols_rc<T, DIR, T, 8, A>(L, v, tw_a, s);
-      return launch_cols_rc<T, DIR, T, 4, A>(L, v, tw_a, s);
+      const int vr = fft_var_ok(L, var) ? var : 0;
+      if (t >= 16) return launch_cols_rc<T, DIR, T, 16, A>(L, v, tw_a, s, vr);
+      if (t == 8) return launch_cols_rc<T, DIR, T, 8, A>(L, v, tw_a, s, vr);
+      return launch_cols_rc<T, DIR, T, 4, A>(L, v, tw_a, s, vr);
     }
   }
-  launch_cols<T, DIR>(L, v, tc, tw_t, s);
+  launch_cols<T, DIR>(L, v, tc, tw_t, s, var);
 }
 
 // In-place y pass over the work grid [n0][n1][ldc].
@@ -197,19 +218,31 @@ void launch_rows_r2c(const FftPlan& p, const Tin* in, int64_t ldi, cplx<T>* out,
     const FftPlan& pt = std::is_same<T, A>::value ? p : *pa;
     const cplx<A>* twMa = pt.tw2M.as<cplx<A>>();
     const cplx<A>* twNa = pt.tw2N.as<cplx<A>>();
-    // Wider arithmetic: at most 8 rows per workgroup (smaller, more
-    // independent workgroups measured faster than the FP32 form's tile).
-    int trc = std::is_same<T, A>::value ? tr : (tr < 8 ? tr : 8);
+    const int var = fft_var_ok(M, p.var[2]) ? p.var[2] : 0;
+    // Wider arithmetic: at most one wave per workgroup (2 rows of M = 256):
+    // small independent workgroups wait at no barrier of other waves
+    // (measured 0.231 -> 0.226 ms for the 512^3 z pass against 8 rows).
+    const int cap = std::max(1, 64 / (M / (var == 1 ? 16 : 8)));
+    int trc = std::is_same<T, A>::value ? tr : (tr < cap ? tr : cap);
     while (trc > 1 && (size_t)trc * row_pad_len(M) * cs > kLdsBudget) trc /= 2;
-#define FC_R2C_RC_CASE(Mv)                                                                   \
-  case Mv: {                                                                                 \
-    const int thr = trc * (Mv / FftCfg<Mv>::E);                                              \
-    dim3 g((unsigned)((nrows + trc - 1) / trc));                                             \
-    hipLaunchKernelGGL((k_fft_rows_r2c_rc<A, Tin, Mv, Loader, T>), g, dim3(thr),             \
-                       (size_t)trc * row_pad_len(Mv) * cs, s, in, ldi, out, ldo, nrows, trc,  \
-                       twMa, twNa, ld);                                                      \
-    break;                                                                                   \
-  }
+    auto go = [&](auto vtag, auto mtag) {
+      constexpr int V = decltype(vtag)::value, Mv = decltype(mtag)::value;
+      dim3 g((unsigned)((nrows + trc - 1) / trc));
+      hipLaunchKernelGGL((k_fft_rows_r2c_rc<A, Tin, Mv, Loader, T, V>), g,
+                         dim3(trc * (Mv / FftCfgV<Mv, V>::E)),
+                         (size_t)trc * row_pad_len(Mv) * cs, s, in, ldi, out, ldo, nrows, trc,
+                         twMa, twNa, ld);
+    };
+#define FC_R2C_RC_CASE(Mv)                                                       \
+  case Mv:                                                                       \
+    if constexpr (FftCfgV<Mv, 1>::fast) {                                        \
+      if (var == 1) {                                                            \
+        go(std::integral_constant<int, 1>{}, std::integral_constant<int, Mv>{}); \
+        break;                                                                   \
+      }                                                                          \
+    }                                                                            \
+    go(std::integral_constant<int, 0>{}, std::integral_constant<int, Mv>{});     \
+    break;
     switch (M) {
       FC_R2C_RC_CASE(8)
       FC_R2C_RC_CASE(16)
@@ -267,15 +300,25 @@ void launch_rows_c2r(const FftPlan& p, const cplx<T>* in, int64_t ldi, Tout* out
     int trc = tr;
     while (trc > 1 && (size_t)trc * row_pad_len(M) * cs > kLdsBudget) trc /= 2;
     const int offm = nsel ? ((off % (2 * M)) + 2 * M) % (2 * M) : 0;
-#define FC_C2R_RC_CASE(Mv)                                                                   \
-  case Mv: {                                                                                 \
-    const int thr = trc * (Mv / FftCfg<Mv>::E);                                              \
-    dim3 g((unsigned)((nrows + trc - 1) / trc));                                             \
-    hipLaunchKernelGGL((k_fft_rows_c2r_rc<T, Tout, Mv>), g, dim3(thr),                       \
-                       (size_t)trc * row_pad_len(Mv) * cs, s, in, ldi, out, ldo, nrows, trc,  \
-                       twM, twN, nsel, offm);                                                \
-    break;                                                                                   \
-  }
+    const int var = fft_var_ok(M, p.var[2]) ? p.var[2] : 0;
+    auto go = [&](auto vtag, auto mtag) {
+      constexpr int V = decltype(vtag)::value, Mv = decltype(mtag)::value;
+      dim3 g((unsigned)((nrows + trc - 1) / trc));
+      hipLaunchKernelGGL((k_fft_rows_c2r_rc<T, Tout, Mv, V>), g,
+                         dim3(trc * (Mv / FftCfgV<Mv, V>::E)),
+                         (size_t)trc * row_pad_len(Mv) * cs, s, in, ldi, out, ldo, nrows, trc,
+                         twM, twN, nsel, offm);
+    };
+#define FC_C2R_RC_CASE(Mv)                                                       \
+  case Mv:                                                                       \
+    if constexpr (FftCfgV<Mv, 1>::fast) {                                        \
+      if (var == 1) {                                                            \
+        go(std::integral_constant<int, 1>{}, std::integral_constant<int, Mv>{}); \
+        break;                                                                   \
+      }                                                                          \
+    }                                                                            \
+    go(std::integral_constant<int, 0>{}, std::integral_constant<int, Mv>{});     \
+    break;
     switch (M) {
       FC_C2R_RC_CASE(8)
       FC_C2R_RC_CASE(16)
@@ -322,15 +365,15 @@ void launch_rows_c2r(const FftPlan& p, const cplx<T>* in, int64_t ldi, Tout* out
 // y then x forward passes over the complex work grid [n0][n1][ldc].
 template <typename T>
 void fft_yx_forward(const FftPlan& p, cplx<T>* data, hipStream_t s) {
-  launch_cols<T, -1>(p.n[1], y_view<T>(p, data), p.tile[1], p.tw1.as<cplx<T>>(), s);
-  launch_cols<T, -1>(p.n[0], x_view<T>(p, data), p.tile[0], p.tw0.as<cplx<T>>(), s);
+  launch_cols<T, -1>(p.n[1], y_view<T>(p, data), p.tile[1], p.tw1.as<cplx<T>>(), s, p.var[1]);
+  launch_cols<T, -1>(p.n[0], x_view<T>(p, data), p.tile[0], p.tw0.as<cplx<T>>(), s, p.var[0]);
 }
 
 // x then y inverse passes.
 template <typename T>
 void fft_xy_inverse(const FftPlan& p, cplx<T>* data, hipStream_t s) {
-  launch_cols<T, +1>(p.n[0], x_view<T>(p, data), p.tile[0], p.tw0.as<cplx<T>>(), s);
-  launch_cols<T, +1>(p.n[1], y_view<T>(p, data), p.tile[1], p.tw1.as<cplx<T>>(), s);
+  launch_cols<T, +1>(p.n[0], x_view<T>(p, data), p.tile[0], p.tw0.as<cplx<T>>(), s, p.var[0]);
+  launch_cols<T, +1>(p.n[1], y_view<T>(p, data), p.tile[1], p.tw1.as<cplx<T>>(), s, p.var[1]);
 }
 
 }  // namespace fc

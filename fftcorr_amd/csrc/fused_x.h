@@ -149,7 +149,7 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
       g[s] = colok ? *reinterpret_cast<const C*>(Bb + o) : C{T(0), T(0)};
     }
   }
-  rstages<T, L, -1, NT, 1, true>(Rad{}, v, lds, L * TC, tw, c, t, Lay{});
+  rstages<T, L, -1, NT, 1, true, 0>(v, lds, L * TC, tw, c, t, Lay{});
 
   // Combine: slot s holds element e = t + out_m(s) L/E of both spectra.
   const int oz = ks.oz();
@@ -201,7 +201,7 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
     }
   }
   __syncthreads();  // the forward pass's last LDS reads precede the inverse's stores
-  rstages<TI, L, +1, 1, 1, true>(Rad{}, u, ldsi, 0, twi, c, t, Lay{});
+  rstages<TI, L, +1, 1, 1, true, 0>(u, ldsi, 0, twi, c, t, Lay{});
   if (colok) {
 #pragma unroll
     for (int s = 0; s < E; ++s) {

@@ -122,10 +122,13 @@ struct BinnedCells {
 // order (optional): within each bin, cells are ordered by (order(cell), cell)
 // instead of raster order -- e.g. by the address the histogram kernel reads
 // for the cell, so that neighbouring entries read neighbouring values.
+// cell_bins (optional): receives the bin of every cell in raster order (-1
+// outside the histogram); needs nbins < 32768.
 void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
                         const std::vector<double>& ay, const std::vector<double>& az,
                         double binsize, int nbins, int64_t track_cell = -1,
-                        const std::function<uint64_t(uint32_t)>* order = nullptr);
+                        const std::function<uint64_t(uint32_t)>* order = nullptr,
+                        std::vector<int16_t>* cell_bins = nullptr);
 
 // Unsigned 32-bit division by a runtime constant d: q = (umulhi(x, m) + x) >> s
 // with the round-up magic m = floor(2^32 (2^s - d) / d) + 1, s = ceil(log2 d)
@@ -260,10 +263,12 @@ struct HistScratch {
 
 // nb batch members: member m's histogram rows start at hist + m * out_bstride
 // (counts likewise).
+// bin_counts (optional): the counts added per bin, instead of bc's own.
 template <int NV, class G>
 void run_hist_seg(G g, const BinnedCells& bc, DeviceBuffer& part, double* hist, int* counts,
                   int row0, int crow0, int crow1, hipStream_t s, int64_t zpos = -1,
-                  double* zval = nullptr, int nb = 1, int64_t out_bstride = 0) {
+                  double* zval = nullptr, int nb = 1, int64_t out_bstride = 0,
+                  const int* bin_counts = nullptr) {
   const int nbins = bc.nbins;
   if (bc.nchunks > 0) {
     part.reserve(sizeof(double) * (size_t)bc.nchunks * NV * nb);
@@ -273,7 +278,8 @@ void run_hist_seg(G g, const BinnedCells& bc, DeviceBuffer& part, double* hist, 
   }
   const int tot = NV * nbins > nbins ? NV * nbins : nbins;
   hipLaunchKernelGGL((k_hist_seg_final<NV>), dim3(tot, nb), dim3(kWave), 0, s,
-                     part.as<double>(), bc.chunk_first.as<int>(), bc.counts.as<int>(), nbins,
+                     part.as<double>(), bc.chunk_first.as<int>(),
+                     bin_counts ? bin_counts : bc.counts.as<int>(), nbins,
                      hist, counts, row0, crow0, crow1, bc.nchunks, out_bstride);
   FC_LAUNCH_CHECK();
 }

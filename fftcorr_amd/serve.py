@@ -79,7 +79,8 @@ def state_bytes(job, precision):
     """Upper bound of the device bytes one geometry keeps resident on a slot:
     the density grid and every buffer of its periodic correlator once it has
     run (work grid, compact k values, pruned inverse arrays, bin-sorted
-    subgrid cells and chunk tables, weights, axes, outputs)."""
+    subgrid cells and chunk tables, the half-subgrid k entries, weights,
+    axes, outputs)."""
     n = job.n
     s = 8 if precision == "fp64" else 4         # grid, forward spectra, k values
     si = 4 if precision == "fp32" else 8        # pruned inverse
@@ -95,7 +96,9 @@ def state_bytes(job, precision):
     cells = 4 * (K + R) + 16 * ((K + R) // 4096 + 2 * nbins + 2) + 8 * (nbins + 1)
     weights = 8 * nv * R + 8 * nv * ((K + R) // 4096 + 2 * nbins + 2)
     small = 8 * 16 * (3 * n + ldc) + 64 * nv * nbins + (1 << 16)
-    return grid + work + pk + inv + cells + weights + small
+    H = ks * ks * (ks // 2 + 1)  # compact k values: the half-subgrid k entries
+    half = 4 * H + 16 * (H // 4096 + 2 * nbins + 2) + 8 * (nbins + 1)
+    return grid + work + pk + inv + cells + weights + small + half
 
 
 def assign_bytes(job):

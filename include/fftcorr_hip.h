@@ -291,16 +291,26 @@ int fc_plan_cache_export(const char* path);
 int fc_plan_cache_import(const char* path);
 int fc_plan_cache_forget(void);
 /* Autotuner (the GPU analogue of FFTW_MEASURE planning, fft_grid.cc:69-115):
- * times the candidate tiles of every pass of a [shape] plan at `precision`
- * (FC_MIXED tunes the FP32 forward plan) on ctx's device, reps timed runs
- * each, and records the fastest in the plan cache (exported by
- * fc_plan_cache_export).  best_ms (may be NULL) receives the best times of
- * the z pass, y pass, x pass and chunked z+y sequence.  Plans created
- * afterwards use the tuned tiles; existing correlators keep theirs. */
+ * times the candidate radix plans and tiles of every pass of a [shape] plan
+ * at `precision` on ctx's device, reps timed runs each, and records the
+ * fastest in the plan cache (exported by fc_plan_cache_export).  FC_MIXED
+ * times the mixed form's kernels (FP32 storage, FP64 arithmetic) and records
+ * its choices for the FP32 plan.  best_ms (may be NULL) receives the best
+ * times of the z pass, y pass, x pass and chunked z+y sequence.  Plans
+ * created afterwards use the tuned choices; existing correlators keep
+ * theirs. */
 int fc_autotune(fc_ctx* ctx, const int shape[3], int precision, int reps, double best_ms[4]);
 /* Tiles a new plan of this shape would use: {x columns, y columns, z rows,
  * z+y chunk KiB (0: unchunked)}. */
 int fc_plan_tiles(int device, const int shape[3], int precision, int tiles[4]);
+/* Radix plan variants a new plan of this shape would use: {x, y, z rows}
+ * (0: first radix 8, 1: first radix 16; the z entry is the row FFT of
+ * length n2/2).  The counterpart of the algorithm FFTW_MEASURE picks. */
+int fc_plan_variants(int device, const int shape[3], int precision, int vars[3]);
+/* The radices of plan variant `var` for an FFT of length L, first to last:
+ * returns their count (at most 8) in *count, 0 when the register-chained
+ * kernels have no such variant for L (other lengths use fixed plans). */
+int fc_fft_radices(int L, int var, int radices[8], int* count);
 
 #ifdef __cplusplus
 }

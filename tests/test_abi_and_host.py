@@ -53,6 +53,41 @@ def test_plan_cache_roundtrip_without_gpu(tmp_path):
         import_wisdom_from_file(str(bad))
 
 
+def test_fft_radix_plans_without_gpu(tmp_path):
+    from fftcorr_amd.autotune import fft_radices
+    from fftcorr_amd.fftw import export_wisdom_to_file, forget_wisdom, import_wisdom_from_file
+    assert fft_radices(512, 0) == [8, 8, 8]
+    assert fft_radices(256, 0) == [8, 8, 4]
+    assert fft_radices(512, 1) == [16, 16, 2]
+    assert fft_radices(256, 1) == [16, 16]
+    assert fft_radices(128, 1) == [16, 8]
+    assert fft_radices(64, 1) is None and fft_radices(384, 0) is None
+    assert fft_radices(512, 2) is None
+    for n in (8, 16, 32, 64, 128, 256, 512):
+        for v in (0, 1):
+            r = fft_radices(n, v)
+            if r is not None:
+                assert int(np.prod(r)) == n and r[0] == (8 if v == 0 else 16)
+    # Variant choices travel with the plan cache; a variant that does not
+    # exist is refused.
+    p = tmp_path / "v.txt"
+    p.write_text("fftcorr_hip_tiles v1\n512 1 3 1\n256 1 4 1\n")
+    import_wisdom_from_file(str(p))
+    out = tmp_path / "out.txt"
+    export_wisdom_to_file(str(out))
+    lines = out.read_text().splitlines()
+    assert "512 1 3 1" in lines and "256 1 4 1" in lines
+    forget_wisdom()
+    bad = tmp_path / "bad.txt"
+    bad.write_text("fftcorr_hip_tiles v1\n512 1 3 2\n")
+    with pytest.raises(IOError):
+        import_wisdom_from_file(str(bad))
+    bad.write_text("fftcorr_hip_tiles v1\n512 1 -1 4\n")
+    with pytest.raises(IOError):
+        import_wisdom_from_file(str(bad))
+    forget_wisdom()
+
+
 def test_grid_construction_and_cell_adoption(capsys):
     # test_grid.py:7-106.
     g = ConfigSpaceGrid([2, 2, 2], [2, -5, 77], [50, 45, 120], cell_size=25,
