@@ -118,6 +118,11 @@ void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
     }
   }
   first[nbins] = (int)chunks.size();
+  std::vector<int> launch(chunks.size());
+  for (size_t q = 0; q < launch.size(); ++q) launch[q] = (int)q;
+  std::stable_sort(launch.begin(), launch.end(), [&](int a, int b) {
+    return cells[chunks[a].y] < cells[chunks[b].y];
+  });
   bc.n = pos;
   bc.nbins = nbins;
   bc.nchunks = (int)chunks.size();
@@ -125,6 +130,7 @@ void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
   bc.counts.allocate(sizeof(int) * nbins);
   bc.chunks.allocate(sizeof(int4) * std::max<size_t>(chunks.size(), 1));
   bc.chunk_first.allocate(sizeof(int) * first.size());
+  bc.order.allocate(sizeof(int) * std::max<size_t>(launch.size(), 1));
   FC_HIP(hipMemcpy(bc.cells.as<void>(), cells.data(), sizeof(uint32_t) * cells.size(),
                    hipMemcpyHostToDevice));
   FC_HIP(hipMemcpy(bc.counts.as<void>(), bc.h_counts.data(), sizeof(int) * nbins,
@@ -134,6 +140,9 @@ void build_binned_cells(BinnedCells& bc, const std::vector<double>& ax,
                      hipMemcpyHostToDevice));
   FC_HIP(hipMemcpy(bc.chunk_first.as<void>(), first.data(), sizeof(int) * first.size(),
                    hipMemcpyHostToDevice));
+  if (!launch.empty())
+    FC_HIP(hipMemcpy(bc.order.as<void>(), launch.data(), sizeof(int) * launch.size(),
+                     hipMemcpyHostToDevice));
 }
 
 }  // namespace fc

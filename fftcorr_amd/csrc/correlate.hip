@@ -1373,6 +1373,7 @@ int fc_corr_footprint(fc_corr* c, int interlaced, size_t* held, size_t* full) {
       add(bc->counts, 0);
       add(bc->chunks, 0);
       add(bc->chunk_first, 0);
+      add(bc->order, 0);
     }
     const fc::FftPlan& p = *c->plan;
     const fc::FftPlan& pi = *c->plan_inv;

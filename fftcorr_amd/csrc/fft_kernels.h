@@ -23,6 +23,8 @@
 
 #include <cstdint>
 
+#include "xcd.h"
+
 namespace fc {
 
 template <typename T>
@@ -549,14 +551,6 @@ void stage_twiddles_(Radices<R, Rest...>, Put put) {
     stage_twiddles_<L, NS * R, (NS > 1 ? OFF + NS * R : OFF)>(Radices<Rest...>{}, put);
 }
 
-// XCD-aware block order: the hardware deals consecutive workgroups to the 8
-// XCDs in turn; give each XCD a contiguous range of tiles so that tiles
-// sharing cache lines meet in one L2.
-__device__ __forceinline__ int xcd_tile(int bid, int nb) {
-  const int q8 = nb >> 3, r8 = nb & 7;
-  const int xcd = bid & 7, slot = bid >> 3;
-  return (xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8) + slot;
-}
 
 // Register-chained strided column pass over [A][L][G][B] (see k_fft_cols),
 // TC columns per workgroup, one launch of A * G * ceil(B / TC) workgroups.

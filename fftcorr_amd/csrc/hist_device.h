@@ -12,6 +12,7 @@
 #include <functional>
 
 #include "fc_objects.h"
+#include "xcd.h"
 
 namespace fc {
 
@@ -110,6 +111,10 @@ struct BinnedCells {
   DeviceBuffer cells;        // uint32 linear index (i*s1 + j)*s2 + k, sorted by bin
   DeviceBuffer counts;       // int32 [nbins]
   DeviceBuffer chunks;       // int4 {bin, first entry, entries, 0} per chunk
+  // int32 [nchunks]: chunks in launch order, by the first cell they hold, so
+  // that chunks of neighbouring bins reading the same cache lines run
+  // together (and, dealt to the XCDs in contiguous ranges, share an L2).
+  DeviceBuffer order;
   DeviceBuffer chunk_first;  // int32 [nbins + 1]: first chunk of each bin
   std::vector<int> h_counts;
   int64_t n = 0;             // entries (cells whose bin is in range)
@@ -158,13 +163,14 @@ struct FastDiv {
 // zero-lag value go to part[m][nchunks][NV] and zval[m].
 template <int NV, class G>
 __global__ void __launch_bounds__(kSegThreads)
-    k_hist_seg(G g, const int4* __restrict__ chunks, double* __restrict__ part, int64_t zpos,
-               double* __restrict__ zval) {
+    k_hist_seg(G g, const int4* __restrict__ chunks, const int* __restrict__ order,
+               double* __restrict__ part, int64_t zpos, double* __restrict__ zval) {
   __shared__ double sw[kSegThreads / kWave][NV];
   g.offset(blockIdx.y);
   part += (size_t)blockIdx.y * gridDim.x * NV;
   if (zval) zval += blockIdx.y;
-  const int4 ch = chunks[blockIdx.x];
+  const int cid = order[xcd_tile(blockIdx.x, gridDim.x)];
+  const int4 ch = chunks[cid];
   double acc[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) acc[q] = 0.0;
@@ -186,7 +192,7 @@ __global__ void __launch_bounds__(kSegThreads)
   if (threadIdx.x < NV) {
     double s = 0.0;
     for (int k = 0; k < kSegThreads / kWave; ++k) s += sw[k][threadIdx.x];
-    part[(size_t)blockIdx.x * NV + threadIdx.x] = s;
+    part[(size_t)cid * NV + threadIdx.x] = s;
   }
 }
 
@@ -273,7 +279,7 @@ void run_hist_seg(G g, const BinnedCells& bc, DeviceBuffer& part, double* hist, 
   if (bc.nchunks > 0) {
     part.reserve(sizeof(double) * (size_t)bc.nchunks * NV * nb);
     hipLaunchKernelGGL((k_hist_seg<NV, G>), dim3(bc.nchunks, nb), dim3(kSegThreads), 0, s, g,
-                       bc.chunks.as<int4>(), part.as<double>(), zpos, zval);
+                       bc.chunks.as<int4>(), bc.order.as<int>(), part.as<double>(), zpos, zval);
     FC_LAUNCH_CHECK();
   }
   const int tot = NV * nbins > nbins ? NV * nbins : nbins;

@@ -93,11 +93,11 @@ def state_bytes(job, precision):
     work = n * n * ldc * 2 * s
     pk = ks * ks * (ks // 2 + 1) * s
     inv = rs * n * ldci * 2 * si + rs * rs * ldci * 2 * si + R * si
-    cells = 4 * (K + R) + 16 * ((K + R) // 4096 + 2 * nbins + 2) + 8 * (nbins + 1)
+    cells = 4 * (K + R) + 20 * ((K + R) // 4096 + 2 * nbins + 2) + 8 * (nbins + 1)
     weights = 8 * nv * R + 8 * nv * ((K + R) // 4096 + 2 * nbins + 2)
     small = 8 * 16 * (3 * n + ldc) + 64 * nv * nbins + (1 << 16)
     H = ks * ks * (ks // 2 + 1)  # compact k values: the half-subgrid k entries
-    half = 4 * H + 16 * (H // 4096 + 2 * nbins + 2) + 8 * (nbins + 1)
+    half = 4 * H + 20 * (H // 4096 + 2 * nbins + 2) + 8 * (nbins + 1)
     return grid + work + pk + inv + cells + weights + small + half
 
 
