@@ -297,6 +297,38 @@ def yamamoto_512(lib, corr_mod, ctx, pos):
                       "headline points in [500, 2500)^3, CIC, kmax 0.5 dk 0.005, rmax 200 dr 5"}
 
 
+def yamamoto_stages(lib, corr_mod, ctx, pos, steps):
+    """The mixed Yamamoto case of yamamoto_512 alone (for profiler runs):
+    wall time per correlation and HIP-event times per stage, on stderr."""
+    c = YAMA_CFG
+    n = c["n"]
+    cell = c["box"] / n
+    posmin = np.full(3, c["posmin"])
+    shifted = np.ascontiguousarray(pos + posmin, np.float64)
+    g = lib.DeviceGrid((n, n, n), posmin, cell, "mixed", ctx)
+    st = lib.MaStats()
+    lib.check(lib.lib().fc_mass_assign(g.handle, lib.dptr(shifted), shifted.shape[0], 3, None,
+                                       1.0, 3, 0, 0.0, st))
+    g.normalize(st.totw / float(n) ** 3)
+    corr = corr_mod.Correlator((n, n, n), cell, posmin, 3, c["rmax"], c["dr"], c["kmax"],
+                               c["dk"], c["maxell"], precision="mixed")
+    lib.check(lib.lib().fc_corr_autocorrelate(corr._handle, g.handle))  # warm-up
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lib.check(lib.lib().fc_corr_autocorrelate(corr._handle, g.handle))
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    corr.set_profiling(True)
+    for _ in range(steps):
+        lib.check(lib.lib().fc_corr_autocorrelate(corr._handle, g.handle))
+    ctx.synchronize()
+    stages = {k: {"ms_per_correlation": v[0] / steps, "launches_per_correlation": v[1] / steps}
+              for k, v in corr.stage_times().items() if v[1]}
+    log("Yamamoto 512^3 mixed: %.3f ms/correlation" % (1e3 * dt))
+    log(json.dumps(stages))
+
+
 def serving(lib, corr_mod):
     """Poisson arrivals (20 jobs/s, seed 5) of periodic xi_l / P_l jobs, l <= 4:
     grids U{128, 256, 384, 512}^3, log-uniform 0.1M-10M uniform or Thomas
@@ -602,6 +634,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="time the mixed form only (for profiler runs); prints no JSON line")
+    ap.add_argument("--yamamoto-only", action="store_true",
+                    help="time the mixed 512^3 Yamamoto case only, with stage times (for "
+                         "profiler runs); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the correlation path here is single-GPU "
@@ -620,6 +655,9 @@ def main():
     pos = np.ascontiguousarray(pos, np.float64)
     log("particles: %d" % pos.shape[0])
 
+    if args.yamamoto_only:
+        yamamoto_stages(lib, corr_mod, ctx, pos, args.steps)
+        return
     side = args.fp64_steps or max(3, args.steps // 4)
     log("mixed form (headline): %d warmup + %d timed steps" % (args.warmup, args.steps))
     fm, gm, cm = run_form(lib, corr_mod, ctx, pos, cfg, "mixed", args.steps, args.warmup,

@@ -48,7 +48,7 @@ struct fc_corr {
   std::vector<double> h_rx, h_kx, h_ky, h_kz;
   fc::DeviceBuffer rx, kx, ky, kz, wx, wy, wz;  // FP64 axes and per-axis window W^2
   fc::BinnedCells kbin, rbin;                   // subgrid cells sorted by bin
-  // Periodic k histogram over the compact values (KEntryHalf): their
+  // k histograms over the compact values (KEntryHalf, KEntryHalfLm): their
   // indices sorted by bin, when the geometry allows it (khalf_ok).
   fc::BinnedCells khalf;
   bool khalf_ok = false;
@@ -241,10 +241,16 @@ struct LoadYlm {
   double c, x0, y0, z0;  // cell centres relative to the observer: x0 + i etc.
   int i0 = 0;            // x slab of row 0
   void shift_x(int dx) { i0 += dx; }
-  __device__ __forceinline__ double operator()(double v, int64_t row, int k) const {
-    const int i = i0 + (int)(row / n1), j = (int)(row % n1);
-    return ylm_eval(ell, m, x0 + (double)i * 1.0, y0 + (double)j * 1.0, z0 + (double)k * 1.0,
-                    c, v);
+  struct Row {
+    int ell, m;
+    double c, x, y, z0;
+    __device__ __forceinline__ double operator()(double v, int k) const {
+      return ylm_eval(ell, m, x, y, z0 + (double)k * 1.0, c, v);
+    }
+  };
+  __device__ __forceinline__ Row row(int64_t r) const {
+    const int i = i0 + (int)(r / n1), j = (int)(r % n1);
+    return Row{ell, m, c, x0 + (double)i * 1.0, y0 + (double)j * 1.0, z0};
   }
 };
 
@@ -344,6 +350,32 @@ struct KEntryHalf {
 #pragma unroll
     for (int q = 0; q < NV; ++q)
       v[q] = (ylm_eval(2 * q, 0, kx[i], ky[j], kz[c], coef[q], invw) * x) * mult;
+  }
+};
+
+// Yamamoto k space over the half-subgrid entries (see KEntryHalf), one (l, m)
+// transform: every real Y_lm of even l is an even polynomial in the unit
+// vector, so a compact value and its mirror cell share the weight.
+template <typename T>
+struct KEntryHalfLm {
+  const uint32_t* cells;
+  const T* pk;
+  FastDiv dz, d1;
+  int nz, ks1;
+  const double *kx, *ky, *kz, *wx, *wy, *wz;  // kz, wz from the kz = 0 plane on
+  int ell, m;
+  double coef;
+  __device__ __forceinline__ void offset(int) {}
+  __device__ __forceinline__ void operator()(int64_t pos, double* v) const {
+#pragma clang fp contract(off)
+    const uint32_t e = cells[pos];
+    const uint32_t t = dz.div(e);
+    const int c = (int)(e - t * (uint32_t)nz);
+    const uint32_t iu = d1.div(t);
+    const int j = (int)(t - iu * (uint32_t)ks1), i = (int)iu;
+    const double x = (double)pk[e];
+    const double invw = 1.0 / (wx[i] * wy[j] * wz[c]);
+    v[0] = (ylm_eval(ell, m, kx[i], ky[j], kz[c], coef, invw) * x) * (c > 0 ? 2.0 : 1.0);
   }
 };
 
@@ -558,7 +590,7 @@ double window_sq(int kind, double k, double cell) {
   return 0;
 }
 
-// Periodic k histogram over the compact values (KEntryHalf): allowed when
+// k histograms over the compact values (KEntryHalf, KEntryHalfLm): allowed when
 // the k subgrid axes are odd (every compact value with c > 0 has its mirror
 // cell) and every mirror pair shares its bin (cb: the bin of every k cell).
 // The entries are then the compact indices of the half subgrid kz >= 0,
@@ -567,7 +599,7 @@ double window_sq(int kind, double k, double cell) {
 void setup_khalf(fc_corr* c, const std::vector<int16_t>& cb) {
   c->khalf_ok = false;
   const int s0 = c->kshape[0], s1 = c->kshape[1], s2 = c->kshape[2];
-  if (c->kind != FC_PERIODIC || s0 % 2 == 0 || s1 % 2 == 0 || s2 % 2 == 0) return;
+  if (s0 % 2 == 0 || s1 % 2 == 0 || s2 % 2 == 0) return;
   const int oz = s2 / 2;
   const int64_t ncols = (int64_t)s0 * s1;
   unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -642,7 +674,7 @@ void setup_geometry(fc_corr* c) {
   };
   {
     std::vector<int16_t> cell_bins;
-    const bool want = c->kind == FC_PERIODIC && c->nkbins < 32768;
+    const bool want = c->nkbins < 32768;
     build_binned_cells(c->kbin, c->h_kx, c->h_ky, c->h_kz, c->dk, c->nkbins, -1, &kkey,
                        want ? &cell_bins : nullptr);
     if (want) setup_khalf(c, cell_bins);
@@ -1212,14 +1244,26 @@ void yamamoto_correlate(fc_corr* c, fc_grid* g1) {
         // k-space multipole l: this transform's share, straight from the
         // compact k values (counts once per l).
         Stage st(c, FC_STAGE_HIST_K);
-        KEntryLm<T> ke{c->kbin.cells.as<uint32_t>(), c->pk.as<T>(), kslots(c),
-                       FastDiv((uint32_t)c->kshape[2]), FastDiv((uint32_t)c->kshape[1]),
-                       c->kx.as<double>(), c->ky.as<double>(), c->kz.as<double>(),
-                       c->wx.as<double>(), c->wy.as<double>(), c->wz.as<double>(), ell, m,
-                       ylm_coeff(ell, m, 4.0 * M_PI)};
         const int row = ell / 2;
-        run_hist_seg<1>(ke, c->kbin, c->hs.part_seg, c->k_hist.as<double>(),
-                        c->k_counts.as<int>(), row, row, m == -ell ? row + 1 : row, s);
+        if (c->khalf_ok) {
+          const int nz = c->kshape[2] / 2 + 1, oz = c->kshape[2] / 2;
+          KEntryHalfLm<T> ke{c->khalf.cells.as<uint32_t>(), c->pk.as<T>(),
+                             FastDiv((uint32_t)nz), FastDiv((uint32_t)c->kshape[1]), nz,
+                             c->kshape[1], c->kx.as<double>(), c->ky.as<double>(),
+                             c->kz.as<double>() + oz, c->wx.as<double>(), c->wy.as<double>(),
+                             c->wz.as<double>() + oz, ell, m, ylm_coeff(ell, m, 4.0 * M_PI)};
+          run_hist_seg<1>(ke, c->khalf, c->hs.part_seg, c->k_hist.as<double>(),
+                          c->k_counts.as<int>(), row, row, m == -ell ? row + 1 : row, s, -1,
+                          nullptr, 1, 0, c->kbin.counts.as<int>());
+        } else {
+          KEntryLm<T> ke{c->kbin.cells.as<uint32_t>(), c->pk.as<T>(), kslots(c),
+                         FastDiv((uint32_t)c->kshape[2]), FastDiv((uint32_t)c->kshape[1]),
+                         c->kx.as<double>(), c->ky.as<double>(), c->kz.as<double>(),
+                         c->wx.as<double>(), c->wy.as<double>(), c->wz.as<double>(), ell, m,
+                         ylm_coeff(ell, m, 4.0 * M_PI)};
+          run_hist_seg<1>(ke, c->kbin, c->hs.part_seg, c->k_hist.as<double>(),
+                          c->k_counts.as<int>(), row, row, m == -ell ? row + 1 : row, s);
+        }
       }
       Stage st(c, FC_STAGE_SUBGRID);
       if (m == -ell) fill<double>(c->rsub.as<double>(), nr, 0.0, s);

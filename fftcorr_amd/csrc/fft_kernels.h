@@ -664,14 +664,15 @@ __global__ void __launch_bounds__(1024)
   {
     using P = typename PairOf<Tin>::type;
     const P* src = reinterpret_cast<const P*>(in + row * ldi);
+    const auto lr = ld.row(row);
 #pragma unroll
     for (int s = 0; s < E; ++s) {
       const int e = t + s * TCOL;
       C z{T(0), T(0)};
       if (rowok) {
         const P p = ld_stream(src + e);
-        z.x = T(ld((double)p.x, row, 2 * e));
-        z.y = T(ld((double)p.y, row, 2 * e + 1));
+        z.x = T(lr((double)p.x, 2 * e));
+        z.y = T(lr((double)p.y, 2 * e + 1));
       }
       v[0][s] = z;
     }
@@ -875,11 +876,15 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---- row (z) pass: real <-> Hermitian half --------------------------------
-// Loader: T operator()(double v, int64 row, int k) turns the input sample
-// (k-th of the row) into the value transformed.  Rows are [nrows][ldi] (input
-// type Tin) and [nrows][ldo] complex outputs.
+// Loader: row(r) gives the per-row functor f with f(double v, int k) the
+// value transformed for the k-th input sample v of row r (the row's own
+// quantities are formed once per thread, not per sample).  Rows are
+// [nrows][ldi] (input type Tin) and [nrows][ldo] complex outputs.
 struct LoadCopy {
-  __device__ __forceinline__ double operator()(double v, int64_t, int) const { return v; }
+  struct Row {
+    __device__ __forceinline__ double operator()(double v, int) const { return v; }
+  };
+  __device__ __forceinline__ Row row(int64_t) const { return Row{}; }
   void shift_x(int) {}  // rows restart at a later x slab: nothing to adjust
 };
 
@@ -909,8 +914,9 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(const Tin* in, int64_t ld
       C z{T(0), T(0)};
       if (row < nrows) {
         const Tin* p = in + row * ldi + 2 * m;
-        z.x = T(ld((double)p[0], row, 2 * m));
-        z.y = T(ld((double)p[1], row, 2 * m + 1));
+        const auto lr = ld.row(row);
+        z.x = T(lr((double)p[0], 2 * m));
+        z.y = T(lr((double)p[1], 2 * m + 1));
       }
       tmp[q] = z;
     }
@@ -1044,8 +1050,9 @@ __global__ void __launch_bounds__(256) k_dft_rows_r2c(const Tin* in, int64_t ldi
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* x = reinterpret_cast<T*>(smem_raw);
   const int64_t row = blockIdx.x;
+  const auto lr = ld.row(row);
   for (int n = threadIdx.x; n < n2; n += blockDim.x)
-    x[n] = T(ld((double)in[row * ldi + n], row, n));
+    x[n] = T(lr((double)in[row * ldi + n], n));
   __syncthreads();
   // Every global read of the row happened before the barrier above, so the
   // output row may alias the input row (in-place transform).
