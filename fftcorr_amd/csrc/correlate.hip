@@ -69,6 +69,11 @@ struct fc_corr {
   fc::HistScratch hs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
+  // Periodic correlations run the k histogram on a second stream (default
+  // priority) beside the pruned inverse; created on first use.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  fc::DeviceBuffer part_k;  // the k histogram's partials (the r histogram's are hs.part_seg)
   // Per-stage profiling: event pairs recorded during the last call(s).
   bool profile = false;
   std::vector<hipEvent_t> ev_pool;
@@ -966,7 +971,7 @@ void gather_pk(fc_corr* c, const cplx<T>* P) {
 // (x lags in the fused kernel, then y and z, in the inverse precision TI)
 // -> r subgrid xisub.
 template <typename T, typename TI>
-void power_and_pruned_inverse(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode) {
+void power_x(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode) {
   hipStream_t s = c->ctx->stream;
   const FftPlan& p = *c->plan;
   const FftPlan& pi = *c->plan_inv;
@@ -992,6 +997,15 @@ void power_and_pruned_inverse(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode
                      (int64_t)p.n[1] * ldq, ldq, 1, p.n[1], p.n2h, rs, o};
     launch_cols<TI, +1, T>(p.n[0], v, pi.tile[0], pi.tw0.as<cplx<TI>>(), s, pi.var[0]);
   }
+}
+
+// Pruned inverse y and z passes: q1 -> q2 -> the r subgrid xisub.
+template <typename T, typename TI>
+void pruned_inverse_yz(fc_corr* c) {
+  hipStream_t s = c->ctx->stream;
+  const FftPlan& pi = *c->plan_inv;
+  const int64_t ldq = pi.ldc;
+  const int rs = c->rshape, o = rs / 2;
   // Pruned inverse y pass: q1[sx][y][c] -> q2[sx][sy][c], only the rs lags.
   {
     Stage st(c, FC_STAGE_FFT_Y_INV);
@@ -1003,6 +1017,14 @@ void power_and_pruned_inverse(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode
   Stage st(c, FC_STAGE_FFT_Z_INV);
   launch_rows_c2r<TI, TI>(pi, c->q2.as<cplx<TI>>(), ldq, c->xisub.as<TI>(), rs, s,
                           (int64_t)rs * rs, rs, o);
+}
+
+// Power spectrum with its compact k values (pk) and pruned inverse x (q1),
+// then the pruned inverse y and z passes (q1 -> xisub).
+template <typename T, typename TI>
+void power_and_pruned_inverse(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode) {
+  power_x<T, TI>(c, w, b, mode);
+  pruned_inverse_yz<T, TI>(c);
 }
 
 // Builds the periodic histogram coefficients and r weights once per correlator.
@@ -1031,30 +1053,68 @@ void ensure_periodic_weights(fc_corr* c) {
   FC_HIP(hipStreamSynchronize(s));  // coef is freed on return
 }
 
-template <typename T, typename TI, int NV>
-void periodic_hists(fc_corr* c) {
-  hipStream_t s = c->ctx->stream;
-  ensure_periodic_weights<NV>(c);
-  {
-    Stage st(c, FC_STAGE_HIST_K);
-    if (c->khalf_ok) {
-      run_hist_khalf<T, NV>(c, c->pk.as<T>(), 0, c->hs.part_seg, c->k_hist.as<double>(),
-                            c->k_counts.as<int>(), 0, 0, NV, s);
-    } else {
-      KEntry<T, NV> ke{c->kbin.cells.as<uint32_t>(), c->pk.as<T>(), kslots(c),
-                       FastDiv((uint32_t)c->kshape[2]), FastDiv((uint32_t)c->kshape[1]),
-                       c->kx.as<double>(), c->ky.as<double>(), c->kz.as<double>(),
-                       c->wx.as<double>(), c->wy.as<double>(), c->wz.as<double>(), {}, 0};
-      for (int q = 0; q < NV; ++q) ke.coef[q] = c->kcoef[q];
-      run_hist_seg<NV>(ke, c->kbin, c->hs.part_seg, c->k_hist.as<double>(),
-                       c->k_counts.as<int>(), 0, 0, NV, s);
-    }
+// Periodic k histogram of pk on stream s, partials in part.
+template <typename T, int NV>
+void periodic_k_hist(fc_corr* c, hipStream_t s, DeviceBuffer& part) {
+  Stage st(c, FC_STAGE_HIST_K);
+  if (c->khalf_ok) {
+    run_hist_khalf<T, NV>(c, c->pk.as<T>(), 0, part, c->k_hist.as<double>(),
+                          c->k_counts.as<int>(), 0, 0, NV, s);
+  } else {
+    KEntry<T, NV> ke{c->kbin.cells.as<uint32_t>(), c->pk.as<T>(), kslots(c),
+                     FastDiv((uint32_t)c->kshape[2]), FastDiv((uint32_t)c->kshape[1]),
+                     c->kx.as<double>(), c->ky.as<double>(), c->kz.as<double>(),
+                     c->wx.as<double>(), c->wy.as<double>(), c->wz.as<double>(), {}, 0};
+    for (int q = 0; q < NV; ++q) ke.coef[q] = c->kcoef[q];
+    run_hist_seg<NV>(ke, c->kbin, part, c->k_hist.as<double>(), c->k_counts.as<int>(), 0, 0,
+                     NV, s);
   }
+}
+
+template <typename TI, int NV>
+void periodic_r_hist(fc_corr* c) {
   Stage st(c, FC_STAGE_HIST_R);
   WEntry<TI, NV> we{c->xisub.as<TI>(), c->rbin.cells.as<uint32_t>(), c->rw.as<double>(),
                     c->rbin.n};
   run_hist_seg<NV>(we, c->rbin, c->hs.part_seg, c->r_hist.as<double>(), c->r_counts.as<int>(), 0,
-                   0, NV, s, c->rbin.tracked_pos, c->zerolag.as<double>());
+                   0, NV, c->ctx->stream, c->rbin.tracked_pos, c->zerolag.as<double>());
+}
+
+// The side stream and its fork / join events (created on first use).
+hipStream_t side_stream(fc_corr* c) {
+  if (!c->side) {
+    FC_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    FC_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    FC_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  }
+  return c->side;
+}
+
+// After the forward passes: power spectrum and pruned inverse x, then the k
+// histogram (which reads only pk) on the side stream while the pruned
+// inverse y and z passes and the r histogram run on the correlator's
+// stream; the stream waits for the side stream at the end, so the next
+// call's clearing of the histograms follows both.  With stage profiling on,
+// everything runs on the correlator's stream, in order.
+template <typename T, typename TI, int NV>
+void periodic_tail(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode) {
+  hipStream_t s = c->ctx->stream;
+  ensure_periodic_weights<NV>(c);
+  power_x<T, TI>(c, w, b, mode);
+  if (c->profile) {
+    periodic_k_hist<T, NV>(c, s, c->part_k);
+    pruned_inverse_yz<T, TI>(c);
+    periodic_r_hist<TI, NV>(c);
+    return;
+  }
+  hipStream_t hs = side_stream(c);
+  FC_HIP(hipEventRecord(c->ev_fork, s));
+  FC_HIP(hipStreamWaitEvent(hs, c->ev_fork, 0));
+  periodic_k_hist<T, NV>(c, hs, c->part_k);
+  FC_HIP(hipEventRecord(c->ev_join, hs));
+  pruned_inverse_yz<T, TI>(c);
+  periodic_r_hist<TI, NV>(c);
+  FC_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
 }
 
 template <typename T, typename TI>
@@ -1072,16 +1132,15 @@ void periodic_correlate(fc_corr* c, fc_grid* g1, int mode, fc_grid* gb) {
   } else if (mode == kModeCross) {
     b = c->grid2_fft.as<cplx<T>>();
   }
-  power_and_pruned_inverse<T, TI>(c, w, b, mode);
   switch (c->nell) {
     case 1:
-      periodic_hists<T, TI, 1>(c);
+      periodic_tail<T, TI, 1>(c, w, b, mode);
       break;
     case 2:
-      periodic_hists<T, TI, 2>(c);
+      periodic_tail<T, TI, 2>(c, w, b, mode);
       break;
     default:
-      periodic_hists<T, TI, 3>(c);
+      periodic_tail<T, TI, 3>(c, w, b, mode);
   }
 }
 
@@ -1391,6 +1450,12 @@ int fc_corr_destroy(fc_corr* corr) {
     if (!corr) return;
     fc::DeviceGuard dg(corr->ctx->device);
     (void)hipStreamSynchronize(corr->ctx->stream);
+    if (corr->side) {
+      (void)hipStreamSynchronize(corr->side);
+      (void)hipStreamDestroy(corr->side);
+    }
+    if (corr->ev_fork) (void)hipEventDestroy(corr->ev_fork);
+    if (corr->ev_join) (void)hipEventDestroy(corr->ev_join);
     for (auto e : corr->ev_pool) (void)hipEventDestroy(e);
     if (corr->ev0) (void)hipEventDestroy(corr->ev0);
     if (corr->ev1) (void)hipEventDestroy(corr->ev1);
@@ -1439,6 +1504,9 @@ int fc_corr_footprint(fc_corr* c, int interlaced, size_t* held, size_t* full) {
     add(c->rw, 8 * nv * (size_t)std::max<int64_t>(c->rbin.n, 1));
     add(c->hs.part_seg,
         8 * nv * (size_t)std::max({c->kbin.nchunks, c->rbin.nchunks, c->khalf.nchunks}));
+    add(c->part_k, c->kind == FC_PERIODIC
+                       ? 8 * nv * (size_t)std::max(c->kbin.nchunks, c->khalf.nchunks)
+                       : 0);
     for (int d = 0; d < 3; ++d)
       add(c->ph[d], sizeof(double2) * (size_t)(d == 2 ? p.ldc : c->shape[d]));
     if (held) *held = now;
