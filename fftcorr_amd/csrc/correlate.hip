@@ -1294,11 +1294,23 @@ void yamamoto_correlate(fc_corr* c, fc_grid* g1) {
   for (int i = 0; i < 3; ++i) start[i] = 0.5 - (-c->posmin[i] / c->cell);
   const int rs = c->rshape;
   const int64_t nr = (int64_t)rs * rs * rs;
+  // Each transform's k histogram runs on the side stream (unless stage
+  // profiling is on) beside its pruned inverse and the next transform's
+  // forward passes; the next power pass, which rewrites pk, waits for it.
+  const bool par = !c->profile;
+  hipStream_t ks = par ? side_stream(c) : s;
+  DeviceBuffer& kpart = par ? c->part_k : c->hs.part_seg;
+  bool pending = false;
   for (int ell = 0; ell <= c->maxell; ell += 2) {
     for (int m = -ell; m <= ell; ++m) {
       LoadYlm ld{ell, m, p.n[1], ylm_coeff(ell, m, 1.0), start[0], start[1], start[2]};
       forward_zy<T, TI>(c, g1, w, ld);
-      power_and_pruned_inverse<T, TI>(c, w, c->grid2_fft.as<cplx<T>>(), kModeCross);
+      if (pending) FC_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
+      power_x<T, TI>(c, w, c->grid2_fft.as<cplx<T>>(), kModeCross);
+      if (par) {
+        FC_HIP(hipEventRecord(c->ev_fork, s));
+        FC_HIP(hipStreamWaitEvent(ks, c->ev_fork, 0));
+      }
       {
         // k-space multipole l: this transform's share, straight from the
         // compact k values (counts once per l).
@@ -1311,19 +1323,24 @@ void yamamoto_correlate(fc_corr* c, fc_grid* g1) {
                              c->kshape[1], c->kx.as<double>(), c->ky.as<double>(),
                              c->kz.as<double>() + oz, c->wx.as<double>(), c->wy.as<double>(),
                              c->wz.as<double>() + oz, ell, m, ylm_coeff(ell, m, 4.0 * M_PI)};
-          run_hist_seg<1>(ke, c->khalf, c->hs.part_seg, c->k_hist.as<double>(),
-                          c->k_counts.as<int>(), row, row, m == -ell ? row + 1 : row, s, -1,
-                          nullptr, 1, 0, c->kbin.counts.as<int>());
+          run_hist_seg<1>(ke, c->khalf, kpart, c->k_hist.as<double>(), c->k_counts.as<int>(),
+                          row, row, m == -ell ? row + 1 : row, ks, -1, nullptr, 1, 0,
+                          c->kbin.counts.as<int>());
         } else {
           KEntryLm<T> ke{c->kbin.cells.as<uint32_t>(), c->pk.as<T>(), kslots(c),
                          FastDiv((uint32_t)c->kshape[2]), FastDiv((uint32_t)c->kshape[1]),
                          c->kx.as<double>(), c->ky.as<double>(), c->kz.as<double>(),
                          c->wx.as<double>(), c->wy.as<double>(), c->wz.as<double>(), ell, m,
                          ylm_coeff(ell, m, 4.0 * M_PI)};
-          run_hist_seg<1>(ke, c->kbin, c->hs.part_seg, c->k_hist.as<double>(),
-                          c->k_counts.as<int>(), row, row, m == -ell ? row + 1 : row, s);
+          run_hist_seg<1>(ke, c->kbin, kpart, c->k_hist.as<double>(), c->k_counts.as<int>(),
+                          row, row, m == -ell ? row + 1 : row, ks);
         }
       }
+      if (par) {
+        FC_HIP(hipEventRecord(c->ev_join, ks));
+        pending = true;
+      }
+      pruned_inverse_yz<T, TI>(c);
       Stage st(c, FC_STAGE_SUBGRID);
       if (m == -ell) fill<double>(c->rsub.as<double>(), nr, 0.0, s);
       hipLaunchKernelGGL(k_acc_r<TI>, dim3(grid1d(nr, 256, 65536)), dim3(256), 0, s,
@@ -1333,6 +1350,7 @@ void yamamoto_correlate(fc_corr* c, fc_grid* g1) {
     }
     hist_subgrids(c, ell, false, true);
   }
+  if (pending) FC_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
 }
 
 void ensure_work(fc_corr* c) { c->work.reserve(c->plan->complex_bytes()); }
@@ -1504,9 +1522,7 @@ int fc_corr_footprint(fc_corr* c, int interlaced, size_t* held, size_t* full) {
     add(c->rw, 8 * nv * (size_t)std::max<int64_t>(c->rbin.n, 1));
     add(c->hs.part_seg,
         8 * nv * (size_t)std::max({c->kbin.nchunks, c->rbin.nchunks, c->khalf.nchunks}));
-    add(c->part_k, c->kind == FC_PERIODIC
-                       ? 8 * nv * (size_t)std::max(c->kbin.nchunks, c->khalf.nchunks)
-                       : 0);
+    add(c->part_k, 8 * nv * (size_t)std::max(c->kbin.nchunks, c->khalf.nchunks));
     for (int d = 0; d < 3; ++d)
       add(c->ph[d], sizeof(double2) * (size_t)(d == 2 ? p.ldc : c->shape[d]));
     if (held) *held = now;
