@@ -8,8 +8,9 @@
 //                     the key cell (the cell holding the particle; for CIC the
 //                     lower corner of its 2x2x2 block of target cells), the
 //                     offsets from it and FP64 weight sums per block;
-//   2. stable radix sort of (key, particle index): within a key cell the
-//      particles keep their input order;
+//   2. stable radix sort of (key, particle index) (radix_sort.h: passes of
+//      at most 8 bits over the bits the largest key needs): within a key
+//      cell the particles keep their input order;
 //   3. k_permute      records in sorted order;
 //   4. k_cell_runs    per occupied key cell, its run {first sorted particle,
 //                     count} (one 8-byte entry) and its bit in an occupancy
@@ -29,12 +30,12 @@
 // No floating-point atomics and no order that depends on scheduling: repeated
 // runs give identical bits.  Each particle record is read by the tiles its
 // stencil reaches (1.3-1.5x on average), never once per target cell.
-#include <hipcub/hipcub.hpp>
 
 #include <mutex>
 #include <vector>
 
 #include "fc_objects.h"
+#include "radix_sort.h"
 
 namespace fc {
 namespace {
@@ -628,24 +629,20 @@ void mass_assign(fc_grid* grid, const double* pos_dev, int64_t n, int ncol,
 
   FC_HIP(hipEventRecord(ev[0], s));
   const int bits = key_bits(sentinel);
-  size_t temp_bytes = 0;
-  FC_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys.as<uint32_t>(),
-                                            keys2.as<uint32_t>(), idx.as<uint32_t>(),
-                                            idx2.as<uint32_t>(), (int)n, 0, bits, s));
-  ctx->scratch[0].reserve(temp_bytes + 256);
-  FC_HIP(hipcub::DeviceRadixSort::SortPairs(ctx->scratch[0].as<void>(), temp_bytes,
-                                            keys.as<uint32_t>(), keys2.as<uint32_t>(),
-                                            idx.as<uint32_t>(), idx2.as<uint32_t>(), (int)n, 0,
-                                            bits, s));
-  hipLaunchKernelGGL(k_permute, dim3(grid1d(n, 256, 65536)), dim3(256), 0, s,
-                     idx2.as<uint32_t>(), rec.as<double4>(), srec.as<double4>(), n);
+  ctx->scratch[0].reserve(radix_plan(n, bits).scratch_bytes());
+  const bool second = radix_sort_pairs(keys.as<uint32_t>(), keys2.as<uint32_t>(),
+                                       idx.as<uint32_t>(), idx2.as<uint32_t>(), n, bits,
+                                       ctx->scratch[0].as<void>(), s);
+  const uint32_t* skeys = second ? keys2.as<uint32_t>() : keys.as<uint32_t>();
+  const uint32_t* sidx = second ? idx2.as<uint32_t>() : idx.as<uint32_t>();
+  hipLaunchKernelGGL(k_permute, dim3(grid1d(n, 256, 65536)), dim3(256), 0, s, sidx,
+                     rec.as<double4>(), srec.as<double4>(), n);
   FC_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_zero_i32, dim3(grid1d(nwords, 256, 65536)), dim3(256), 0, s,
                      occ.as<int32_t>(), (int64_t)nwords);
   FC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_cell_runs, dim3(grid1d(n, 256, 65536)), dim3(256), 0, s,
-                     keys2.as<uint32_t>(), partn.as<unsigned long long>() + nbp, runs.as<int2>(),
-                     occ.as<uint32_t>());
+  hipLaunchKernelGGL(k_cell_runs, dim3(grid1d(n, 256, 65536)), dim3(256), 0, s, skeys,
+                     partn.as<unsigned long long>() + nbp, runs.as<int2>(), occ.as<uint32_t>());
   FC_LAUNCH_CHECK();
   FC_HIP(hipEventRecord(ev[1], s));
   if (grid->prec == FC_FP64)
@@ -713,11 +710,8 @@ int fc_mass_assign_footprint(fc_grid* grid, int64_t n, int ncol, int has_weights
     if (n > 0) {
       const uint64_t ncells = grid->size();
       const int nbp = (int)std::min<int64_t>(1024, (n + fc::kPrepThreads - 1) / fc::kPrepThreads);
-      size_t temp = 0;
-      FC_HIP(hipcub::DeviceRadixSort::SortPairs(
-          nullptr, temp, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
-          (uint32_t*)nullptr, (int)n, 0, fc::key_bits(ncells), ctx->stream));
-      const size_t want[12] = {temp + 256,
+      const size_t temp = fc::radix_plan(n, fc::key_bits(ncells)).scratch_bytes();
+      const size_t want[12] = {temp,
                                sizeof(uint32_t) * n,
                                sizeof(uint32_t) * n,
                                sizeof(uint32_t) * n,
