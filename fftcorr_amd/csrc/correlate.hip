@@ -995,7 +995,7 @@ void power_x(fc_corr* c, cplx<T>* w, const cplx<T>* b, int mode) {
     Stage st(c, FC_STAGE_FFT_X_INV);
     ColView<TI, T> v{w, 0, (int64_t)p.n[1] * p.ldc, (int64_t)p.ldc, c->q1.as<cplx<TI>>(), 0,
                      (int64_t)p.n[1] * ldq, ldq, 1, p.n[1], p.n2h, rs, o};
-    launch_cols<TI, +1, T>(p.n[0], v, pi.tile[0], pi.tw0.as<cplx<TI>>(), s, pi.var[0]);
+    launch_cols<TI, +1, T>(p.n[0], v, pi.tile[0], pi.tw0.as<cplx<TI>>(), s);
   }
 }
 
@@ -1011,7 +1011,7 @@ void pruned_inverse_yz(fc_corr* c) {
     Stage st(c, FC_STAGE_FFT_Y_INV);
     ColView<TI> v{c->q1.as<cplx<TI>>(), (int64_t)pi.n[1] * ldq, ldq, 0, c->q2.as<cplx<TI>>(),
                   (int64_t)rs * ldq, ldq, 0, rs, 1, pi.n2h, rs, o};
-    launch_cols<TI, +1>(pi.n[1], v, pi.tile[1], pi.tw1.as<cplx<TI>>(), s, pi.var[1]);
+    launch_cols<TI, +1>(pi.n[1], v, pi.tile[1], pi.tw1.as<cplx<TI>>(), s);
   }
   // Pruned z C2R: rows of q2 -> r subgrid rows xi[sx][sy][sz].
   Stage st(c, FC_STAGE_FFT_Z_INV);
@@ -1152,11 +1152,11 @@ T* full_inverse(fc_corr* c, cplx<T>* w) {
   hipStream_t s = c->ctx->stream;
   {
     Stage st(c, FC_STAGE_FFT_X_INV);
-    launch_cols<T, +1>(p.n[0], x_view<T>(p, w), p.tile[0], p.tw0.as<cplx<T>>(), s, p.var[0]);
+    launch_cols<T, +1>(p.n[0], x_view<T>(p, w), p.tile[0], p.tw0.as<cplx<T>>(), s);
   }
   {
     Stage st(c, FC_STAGE_FFT_Y_INV);
-    launch_cols<T, +1>(p.n[1], y_view<T>(p, w), p.tile[1], p.tw1.as<cplx<T>>(), s, p.var[1]);
+    launch_cols<T, +1>(p.n[1], y_view<T>(p, w), p.tile[1], p.tw1.as<cplx<T>>(), s);
   }
   Stage st(c, FC_STAGE_FFT_Z_INV);
   T* xr = reinterpret_cast<T*>(w);
@@ -1769,13 +1769,13 @@ void batch_sequence(fc_batch* bt, int n) {
       ColView<TI, T> v{wi, 0, (int64_t)p.n[1] * p.ldc, (int64_t)p.ldc,
                        bt->q1.as<cplx<TI>>() + i * bt->nq1, 0, (int64_t)p.n[1] * ldq, ldq, 1,
                        p.n[1], p.n2h, rs, o};
-      launch_cols<TI, +1, T>(p.n[0], v, pi.tile[0], pi.tw0.as<cplx<TI>>(), s, pi.var[0]);
+      launch_cols<TI, +1, T>(p.n[0], v, pi.tile[0], pi.tw0.as<cplx<TI>>(), s);
     }
   }
   {
     ColView<TI> v{bt->q1.as<cplx<TI>>(), (int64_t)pi.n[1] * ldq, ldq, 0, bt->q2.as<cplx<TI>>(),
                   (int64_t)rs * ldq, ldq, 0, n * rs, 1, pi.n2h, rs, o};
-    launch_cols<TI, +1>(pi.n[1], v, pi.tile[1], pi.tw1.as<cplx<TI>>(), s, pi.var[1]);
+    launch_cols<TI, +1>(pi.n[1], v, pi.tile[1], pi.tw1.as<cplx<TI>>(), s);
   }
   launch_rows_c2r<TI, TI>(pi, bt->q2.as<cplx<TI>>(), ldq, bt->xisub.as<TI>(), rs, s,
                           (int64_t)n * rs * rs, rs, o);

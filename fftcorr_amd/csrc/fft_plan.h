@@ -34,7 +34,8 @@ struct FftPlan {
   bool fast[3] = {false, false, false};  // axis 2: the half-length row FFT
   int tile[3] = {0, 0, 0};
   // Register-chained plan variant per axis (FftCfgV; axis 2: the row FFT of
-  // length n2/2), 0 unless the autotuner or imported wisdom chose another.
+  // length n2/2) of the forward passes, which the autotuner times; 0 unless
+  // it or imported wisdom chose another.  Inverse passes use variant 0.
   int var[3] = {0, 0, 0};
   // x slabs per z+y chunk: the y pass of a chunk reads the z pass's output
   // while it is still in the last-level cache (0: the whole grid at once).
@@ -300,25 +301,15 @@ void launch_rows_c2r(const FftPlan& p, const cplx<T>* in, int64_t ldi, Tout* out
     int trc = tr;
     while (trc > 1 && (size_t)trc * row_pad_len(M) * cs > kLdsBudget) trc /= 2;
     const int offm = nsel ? ((off % (2 * M)) + 2 * M) % (2 * M) : 0;
-    const int var = fft_var_ok(M, p.var[2]) ? p.var[2] : 0;
-    auto go = [&](auto vtag, auto mtag) {
-      constexpr int V = decltype(vtag)::value, Mv = decltype(mtag)::value;
-      dim3 g((unsigned)((nrows + trc - 1) / trc));
-      hipLaunchKernelGGL((k_fft_rows_c2r_rc<T, Tout, Mv, V>), g,
-                         dim3(trc * (Mv / FftCfgV<Mv, V>::E)),
-                         (size_t)trc * row_pad_len(Mv) * cs, s, in, ldi, out, ldo, nrows, trc,
-                         twM, twN, nsel, offm);
-    };
-#define FC_C2R_RC_CASE(Mv)                                                       \
-  case Mv:                                                                       \
-    if constexpr (FftCfgV<Mv, 1>::fast) {                                        \
-      if (var == 1) {                                                            \
-        go(std::integral_constant<int, 1>{}, std::integral_constant<int, Mv>{}); \
-        break;                                                                   \
-      }                                                                          \
-    }                                                                            \
-    go(std::integral_constant<int, 0>{}, std::integral_constant<int, Mv>{});     \
-    break;
+    // The radix-8 plans (plan variants are tuned on the forward passes).
+#define FC_C2R_RC_CASE(Mv)                                                                   \
+  case Mv: {                                                                                 \
+    dim3 g((unsigned)((nrows + trc - 1) / trc));                                             \
+    hipLaunchKernelGGL((k_fft_rows_c2r_rc<T, Tout, Mv>), g, dim3(trc * (Mv / FftCfg<Mv>::E)), \
+                       (size_t)trc * row_pad_len(Mv) * cs, s, in, ldi, out, ldo, nrows, trc,  \
+                       twM, twN, nsel, offm);                                                \
+    break;                                                                                   \
+  }
     switch (M) {
       FC_C2R_RC_CASE(8)
       FC_C2R_RC_CASE(16)
@@ -372,8 +363,8 @@ void fft_yx_forward(const FftPlan& p, cplx<T>* data, hipStream_t s) {
 // x then y inverse passes.
 template <typename T>
 void fft_xy_inverse(const FftPlan& p, cplx<T>* data, hipStream_t s) {
-  launch_cols<T, +1>(p.n[0], x_view<T>(p, data), p.tile[0], p.tw0.as<cplx<T>>(), s, p.var[0]);
-  launch_cols<T, +1>(p.n[1], y_view<T>(p, data), p.tile[1], p.tw1.as<cplx<T>>(), s, p.var[1]);
+  launch_cols<T, +1>(p.n[0], x_view<T>(p, data), p.tile[0], p.tw0.as<cplx<T>>(), s);
+  launch_cols<T, +1>(p.n[1], y_view<T>(p, data), p.tile[1], p.tw1.as<cplx<T>>(), s);
 }
 
 }  // namespace fc
