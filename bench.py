@@ -75,9 +75,11 @@ def make_grids(lib, ctx, pos, cfg, precision):
     return grids, ma_ms, st
 
 
-def stage_bytes(cfg, s_work, s_grid, kshape, rs, s_inv=None, k_entries=None):
+def stage_bytes(cfg, s_work, s_grid, kshape, rs, s_inv=None, k_entries=None, fwd_grids=1):
     """Algorithmic HBM bytes per launch of each stage: the data each stage
-    must read and write once.  Complex half grid n^2 (n/2+1) elements of
+    must read and write once.  The forward z and y passes transform
+    fwd_grids grids per launch (2 when an interlaced pair runs as one launch
+    per pass).  Complex half grid n^2 (n/2+1) elements of
     2 s_work bytes; real grid n^3 of s_grid; compact k-subgrid values
     ks0 ks1 (ks2/2+1) of s_work; pruned inverse arrays rs n (n/2+1) and
     rs^2 (n/2+1) complex and the r subgrid rs^3 real, of s_inv bytes.  The k
@@ -94,8 +96,8 @@ def stage_bytes(cfg, s_work, s_grid, kshape, rs, s_inv=None, k_entries=None):
     q2 = rs * rs * nh * 2 * s_inv
     xs = rs ** 3 * s_inv
     return {
-        "fft_z_fwd": n ** 3 * s_grid + half,
-        "fft_y_fwd": 2 * half,
+        "fft_z_fwd": fwd_grids * (n ** 3 * s_grid + half),
+        "fft_y_fwd": fwd_grids * 2 * half,
         "combine": 2 * half + pk + q1,  # fused x pass: read a, b; write pk, pruned q1
         "hist_k": (k_entries if k_entries is not None else pk // s_work) * (4 + s_work),
         "fft_y_inv": q1 + q2,
@@ -708,9 +710,11 @@ def main():
                          for k in ("k_hist", "r_hist"))
 
     # Roofline of the dominant stage (HIP events on the correlator's stream).
-    sb = stage_bytes(cfg, 4, 4, fm["kshape"], fm["rshape"][0], s_inv=8,
-                     k_entries=k_half_entries(cfg, fm["kshape"]))
     st = {k: v for k, v in fm["stages"].items() if v[1] > 0}
+    # Two grids per correlation: one forward launch per pass covers both.
+    fwd_grids = max(1, round(2 * fm["steps"] / st["fft_z_fwd"][1]))
+    sb = stage_bytes(cfg, 4, 4, fm["kshape"], fm["rshape"][0], s_inv=8,
+                     k_entries=k_half_entries(cfg, fm["kshape"]), fwd_grids=fwd_grids)
     dom = max(st, key=lambda k: st[k][0])
     avg_ms = st[dom][0] / st[dom][1]
     achieved = sb[dom] / (avg_ms * 1e-3) / 1e9 if dom in sb else None

@@ -643,11 +643,13 @@ struct PairOf<double> {
 // values, a length-M FFT, then the Hermitian post-process (see
 // k_fft_rows_r2c).  Block = tr rows x M/E threads, LDS tr row_pad_len(M).
 // Arithmetic in T; LDS and outputs in To.
+// in2 (optional): rows from nrows1 on come from in2 (row - nrows1), so one
+// launch transforms two grids into consecutive output rows.
 template <typename T, typename Tin, int M, class Loader, typename To = T, int V = 0>
 __global__ void __launch_bounds__(1024)
     k_fft_rows_r2c_rc(const Tin* in, int64_t ldi, cplx<To>* out, int64_t ldo, int64_t nrows,
                       int tr, const cplx<T>* __restrict__ twM, const cplx<T>* __restrict__ twN,
-                      Loader ld) {
+                      Loader ld, const Tin* in2 = nullptr, int64_t nrows1 = 0) {
   using C = cplx<T>;
   constexpr int E = FftCfgV<M, V>::E;
   constexpr int TCOL = M / E;
@@ -663,7 +665,8 @@ __global__ void __launch_bounds__(1024)
   C v[1][E];
   {
     using P = typename PairOf<Tin>::type;
-    const P* src = reinterpret_cast<const P*>(in + row * ldi);
+    const P* src = reinterpret_cast<const P*>(
+        in2 && row >= nrows1 ? in2 + (row - nrows1) * ldi : in + row * ldi);
     const auto lr = ld.row(row);
 #pragma unroll
     for (int s = 0; s < E; ++s) {

@@ -206,15 +206,20 @@ ColView<T> x_view(const FftPlan& p, cplx<T>* d) {
 // z pass (R2C rows) stored in T.  With A wider than T, the register-chained
 // kernel computes in A with the twiddles of the A-precision plan *pa (same
 // shape), exchanging through LDS in T; lengths it does not take compute in T.
+// in2 / nrows1 (optional): rows from nrows1 on are read from in2 (a second
+// grid of the same shape), register-chained row lengths only.
 template <typename T, typename Tin, class Loader, typename A = T>
 void launch_rows_r2c(const FftPlan& p, const Tin* in, int64_t ldi, cplx<T>* out, int64_t ldo,
-                     Loader ld, hipStream_t s, int64_t nrows = -1, const FftPlan* pa = nullptr) {
+                     Loader ld, hipStream_t s, int64_t nrows = -1, const FftPlan* pa = nullptr,
+                     const Tin* in2 = nullptr, int64_t nrows1 = 0) {
   const size_t cs = sizeof(cplx<T>);
   if (nrows < 0) nrows = p.nrows();
   const int M = p.n[2] / 2;
   const int tr = p.tile[2];
   const cplx<T>* twM = p.tw2M.as<cplx<T>>();
   const cplx<T>* twN = p.tw2N.as<cplx<T>>();
+  if (in2 && !(p.fast[2] && rc_len_ok(M)))
+    fail(FC_ERR_RUNTIME, "internal: two-grid z pass needs the register-chained rows");
   if (p.fast[2] && rc_len_ok(M)) {
     const FftPlan& pt = std::is_same<T, A>::value ? p : *pa;
     const cplx<A>* twMa = pt.tw2M.as<cplx<A>>();
@@ -232,7 +237,7 @@ void launch_rows_r2c(const FftPlan& p, const Tin* in, int64_t ldi, cplx<T>* out,
       hipLaunchKernelGGL((k_fft_rows_r2c_rc<A, Tin, Mv, Loader, T, V>), g,
                          dim3(trc * (Mv / FftCfgV<Mv, V>::E)),
                          (size_t)trc * row_pad_len(Mv) * cs, s, in, ldi, out, ldo, nrows, trc,
-                         twMa, twNa, ld);
+                         twMa, twNa, ld, in2, nrows1);
     };
 #define FC_R2C_RC_CASE(Mv)                                                       \
   case Mv:                                                                       \
