@@ -425,8 +425,21 @@ __host__ __device__ constexpr int row_pad_len(int M) { return M + M / 8 + 1; }
 // of the axis (see stage_twiddle_len): the stage with prefix NS owns NS * R
 // entries at OFF, entry k * R + r = exp(-2 pi i r k / (NS R)), so a thread's
 // R - 1 twiddles of one stage are contiguous.
+// Barrier policies of the register-chained stages.  BarWg: __syncthreads().
+// BarLds: waits for this wave's LDS accesses only and joins the workgroup,
+// for kernels that keep direct-to-LDS prefetches of other LDS in flight
+// across the stages (and wait for those explicitly).
+struct BarWg {
+  __device__ __forceinline__ static void sync() { __syncthreads(); }
+};
+struct BarLds {
+  __device__ __forceinline__ static void sync() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  }
+};
+
 template <typename T, int L, int R, int NS, int OFF, int DIR, int NT, bool FIRST, bool LAST,
-          class Lay, int Q0, int NQ, class CL, int V>
+          class Lay, int Q0, int NQ, class CL, int V, class Bar = BarWg>
 __device__ __forceinline__ void rstage(cplx<T> (&v)[NT][(FftCfgV<L, V>::E)], CL* lds, int ts,
                                        const cplx<T>* __restrict__ tws, int c, int t, Lay lay) {
   using TL = typename ScalarOf<CL>::type;
@@ -443,7 +456,7 @@ __device__ __forceinline__ void rstage(cplx<T> (&v)[NT][(FftCfgV<L, V>::E)], CL*
 #pragma unroll
         for (int r = 0; r < R; ++r)
           v[q][b * R + r] = to_cplx<T>(lds[(q - Q0) * ts + lay(c, t + b * TCOL + r * LR)]);
-    if (!LAST) __syncthreads();
+    if (!LAST) Bar::sync();
   }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -485,23 +498,23 @@ __device__ __forceinline__ void rstage(cplx<T> (&v)[NT][(FftCfgV<L, V>::E)], CL*
           lds[(q - Q0) * ts + lay(c, base + r * NS)] = to_cplx<TL>(v[q][b * R + r]);
     }
   }
-  if (!LAST) __syncthreads();
+  if (!LAST) Bar::sync();
 }
 
 template <typename T, int L, int DIR, int NT, int NS, int OFF, bool FIRST, int Q0, int NQ,
-          class Lay, class CL, int V>
+          class Lay, class CL, int V, class Bar>
 __device__ __forceinline__ void rstages_(Radices<>, cplx<T> (&)[NT][(FftCfgV<L, V>::E)], CL*,
                                          int, const cplx<T>*, int, int, Lay) {}
 
 template <typename T, int L, int DIR, int NT, int NS, int OFF, bool FIRST, int Q0, int NQ,
-          class Lay, class CL, int V, int R, int... Rest>
+          class Lay, class CL, int V, class Bar, int R, int... Rest>
 __device__ __forceinline__ void rstages_(Radices<R, Rest...>,
                                          cplx<T> (&v)[NT][(FftCfgV<L, V>::E)], CL* lds, int ts,
                                          const cplx<T>* __restrict__ tws, int c, int t,
                                          Lay lay) {
-  rstage<T, L, R, NS, OFF, DIR, NT, FIRST, sizeof...(Rest) == 0, Lay, Q0, NQ, CL, V>(
+  rstage<T, L, R, NS, OFF, DIR, NT, FIRST, sizeof...(Rest) == 0, Lay, Q0, NQ, CL, V, Bar>(
       v, lds, ts, tws, c, t, lay);
-  rstages_<T, L, DIR, NT, NS * R, (NS > 1 ? OFF + NS * R : OFF), false, Q0, NQ, Lay, CL, V>(
+  rstages_<T, L, DIR, NT, NS * R, (NS > 1 ? OFF + NS * R : OFF), false, Q0, NQ, Lay, CL, V, Bar>(
       Radices<Rest...>{}, v, lds, ts, tws, c, t, lay);
 }
 
@@ -535,10 +548,10 @@ constexpr int stage_twiddle_base() {
 // All stages of plan variant V; tw is the axis's twiddle table.  Tiles
 // Q0 .. Q0 + NQ - 1 of v run in lock-step (LDS tile q - Q0).
 template <typename T, int L, int DIR, int NT, int NS, bool FIRST, int V, class Lay, int Q0 = 0,
-          int NQ = NT, class CL>
+          int NQ = NT, class Bar = BarWg, class CL>
 __device__ __forceinline__ void rstages(cplx<T> (&v)[NT][(FftCfgV<L, V>::E)], CL* lds, int ts,
                                         const cplx<T>* __restrict__ tw, int c, int t, Lay lay) {
-  rstages_<T, L, DIR, NT, NS, 0, FIRST, Q0, NQ, Lay, CL, V>(
+  rstages_<T, L, DIR, NT, NS, 0, FIRST, Q0, NQ, Lay, CL, V, Bar>(
       typename FftCfgV<L, V>::radices{}, v, lds, ts, tw + L + stage_twiddle_base<L, V>(), c, t,
       lay);
 }
@@ -613,6 +626,150 @@ __attribute__((amdgpu_waves_per_eu(rc_wide_waves<T, To, V>())))
     for (; o < nout; o += L)
       st_stream(reinterpret_cast<cplx<To>*>(dstb + (cout + (uint32_t)o * outLb)),
                 to_cplx<To>(v[0][s]));
+  }
+}
+
+// ---- persistent column pass with direct-to-LDS prefetch ---------------------
+// The column pass above holds two workgroups (32 waves) per CU, each loading
+// its tile, transforming it through LDS and storing it in turn; with both
+// often in the same phase, HBM idles while they compute.  This form runs one
+// workgroup per CU over a sequence of tiles: while it
+// transforms one tile in one LDS buffer, the next streams into the other with
+// global_load_lds_dwordx4 (no registers held), and the stage barriers wait
+// only for LDS (BarLds) so the prefetch stays in flight across them.  Tiles
+// are TC columns of 128-byte rows (16 FP32 / 8 FP64 complex); the stage
+// twiddles sit in LDS, so no other global load is issued behind the prefetch.
+// The stages and twiddles are those of k_fft_cols_rc; the compiler may fuse
+// their multiplies and adds differently, so an FP32 output can differ from
+// k_fft_cols_rc's in its last bit (2 of 268M values over a 512^3 pair).
+
+__device__ __forceinline__ uint32_t lds_u32(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+
+// 16 bytes per lane from gbase + lane_off (gbase uniform) into LDS at
+// lds + 16 * lane.  M0 (the LDS base of the transfer) is saved and restored;
+// the wave waits for the transfer with an explicit vmcnt(0).  No outputs.
+__device__ __forceinline__ void lds_dma16(const char* gbase, uint32_t lane_off, uint32_t lds) {
+  uint32_t saved;
+  asm volatile(
+      "s_mov_b32 %0, m0\n\t"
+      "s_mov_b32 m0, %1\n\t"
+      "s_nop 0\n\t"
+      "global_load_lds_dwordx4 %2, %3\n\t"
+      "s_nop 0\n\t"
+      "s_mov_b32 m0, %0"
+      : "=&s"(saved)
+      : "s"(lds), "v"(lane_off), "s"(gbase)
+      : "memory");
+}
+
+__device__ __forceinline__ void wait_vm_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// A tile (L rows of 128 bytes from src, row pitch rowb bytes) into buf:
+// block k of 1 KB holds rows 8k .. 8k + 7, one wave instruction each.
+template <int L, int TC>
+__device__ __forceinline__ void pipe_fetch(const char* src, uint32_t rowb, void* buf) {
+  constexpr int kWaves = TC * (L / 8) / 64;
+  constexpr int kPer = (L / 8) / kWaves;  // blocks per wave
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64);
+  const int l = threadIdx.x % 64;
+  const uint32_t lane_off = (uint32_t)(l / 8) * rowb + (uint32_t)(l % 8) * 16u;
+  const uint32_t lb = lds_u32(buf);
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int blk = w * kPer + j;
+    lds_dma16(src + (uint64_t)(blk * 8) * rowb, lane_off,
+              __builtin_amdgcn_readfirstlane(lb + (uint32_t)blk * 1024u));
+  }
+}
+
+template <int L, int TC, typename Ts>
+constexpr size_t pipe_lds_bytes() {
+  return (size_t)L * TC * sizeof(cplx<Ts>);
+}
+
+// One tile: its raw rows are in cur; after the transform cur is refilled
+// with tile tt + 2 G (G = gridDim.x; the other buffer already holds tile
+// tt + G).
+template <typename T, int L, int DIR, int TC, typename Ts>
+__device__ __forceinline__ void pipe_tile(const cplx<Ts>* in, int64_t inA, uint32_t inLb,
+                                          cplx<Ts>* out, int64_t outA, uint32_t outLb, int B,
+                                          int tilesB, int ntiles, int tt, const cplx<T>* twl,
+                                          cplx<Ts>* cur) {
+  using C = cplx<T>;
+  constexpr int E = FftCfgV<L, 0>::E;
+  constexpr int TCOL = L / E;
+  const int c = threadIdx.x % TC, t = threadIdx.x / TC;
+  C v[1][E];
+#pragma unroll
+  for (int s = 0; s < E; ++s) v[0][s] = to_cplx<T>(cur[(t + s * TCOL) * TC + c]);
+  BarLds::sync();  // the raw tile is in registers: cur takes the exchanges
+  rstages<T, L, DIR, 1, 1, true, 0, LaySwz<TC, E>, 0, 1, BarLds>(v, cur, 0, twl, c, t,
+                                                                 LaySwz<TC, E>{});
+  wait_vm_all();   // this wave's prefetch of tile tt + G (and its earlier stores) done
+  BarLds::sync();  // every wave's; and every wave has read cur for the last time
+  const int nt = tt + 2 * (int)gridDim.x;
+  if (nt < ntiles) {
+    const int a = nt / tilesB;
+    const int b0 = (nt - a * tilesB) * TC;
+    pipe_fetch<L, TC>(reinterpret_cast<const char*>(in + (int64_t)a * inA + b0), inLb, cur);
+  }
+  const int a = tt / tilesB;
+  const int b0 = (tt - a * tilesB) * TC;
+  if (b0 + c < B) {
+    char* dstb = reinterpret_cast<char*>(out + (int64_t)a * outA + b0);
+    const uint32_t lane_off = (uint32_t)c * (uint32_t)sizeof(cplx<Ts>) + (uint32_t)t * outLb;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      char* rowp = dstb + (uint64_t)(out_m<L, 0>(s) * TCOL) * outLb;  // uniform
+      st_stream(reinterpret_cast<cplx<Ts>*>(rowp + lane_off), to_cplx<Ts>(v[0][s]));
+    }
+  }
+}
+
+// Column pass over [A][L][B] (slab stride inA / outA, row pitch inL / outL
+// complex elements of Ts), all L outputs, plan variant 0.  Launch
+// min(ntiles, resident workgroups) workgroups of TC * L / 8 threads;
+// workgroup b takes tiles b, b + G, ... (G = gridDim.x), so the chip works
+// through the grid slab by slab (a contiguous range per workgroup measured
+// 10 % slower).  tw is the axis table (plain part, then the stage table).
+// The launcher checks that each tile's 128-byte rows lie inside the rows
+// (tilesB * TC <= inL).
+template <typename T, int L, int DIR, int TC, typename Ts>
+__global__ void __launch_bounds__(TC*(L / 8))
+    k_fft_cols_pipe(const cplx<Ts>* in, int64_t inA, int64_t inL, cplx<Ts>* out, int64_t outA,
+                    int64_t outL, int B, int tilesB, int ntiles, const cplx<T>* __restrict__ tw) {
+  static_assert(FftCfgV<L, 0>::fast && FftCfgV<L, 0>::E == 8, "register-chained plan of E = 8");
+  static_assert(TC * sizeof(cplx<Ts>) == 128, "tiles of 128-byte rows");
+  constexpr int TWN = L + stage_twiddle_len<L, 0>();
+  __shared__ __align__(16) cplx<Ts> bufA[L * TC];
+  __shared__ __align__(16) cplx<Ts> bufB[L * TC];
+  __shared__ __align__(16) cplx<T> twl[TWN];
+  const uint32_t inLb = (uint32_t)inL * (uint32_t)sizeof(cplx<Ts>);
+  const uint32_t outLb = (uint32_t)outL * (uint32_t)sizeof(cplx<Ts>);
+  const int G = gridDim.x;
+  int tt = blockIdx.x;
+  if (tt >= ntiles) return;
+  auto fetch = [&](int k, cplx<Ts>* buf) {
+    const int a = k / tilesB;
+    const int b0 = (k - a * tilesB) * TC;
+    pipe_fetch<L, TC>(reinterpret_cast<const char*>(in + (int64_t)a * inA + b0), inLb, buf);
+  };
+  fetch(tt, bufA);
+  for (int i = L + (int)threadIdx.x; i < TWN; i += blockDim.x) twl[i] = tw[i];
+  wait_vm_all();
+  __syncthreads();
+  if (tt + G < ntiles) fetch(tt + G, bufB);
+  for (;;) {
+    pipe_tile<T, L, DIR, TC, Ts>(in, inA, inLb, out, outA, outLb, B, tilesB, ntiles, tt, twl,
+                                 bufA);
+    tt += G;
+    if (tt >= ntiles) break;
+    pipe_tile<T, L, DIR, TC, Ts>(in, inA, inLb, out, outA, outLb, B, tilesB, ntiles, tt, twl,
+                                 bufB);
+    tt += G;
+    if (tt >= ntiles) break;
   }
 }
 

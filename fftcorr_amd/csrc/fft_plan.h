@@ -6,6 +6,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <type_traits>
 
 #include "fc_internal.h"
@@ -83,8 +84,48 @@ void launch_cols_rc_v(const ColView<T, Tin>& v, int64_t nb, int tilesB, int nout
                      nout, offm, tw);
 }
 
+// Compute units of the current device (queried once per device).
+inline int current_device_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  FC_HIP(hipGetDevice(&dev));
+  int n = dev >= 0 && dev < 64 ? cache[dev].load(std::memory_order_relaxed) : 0;
+  if (n <= 0) {
+    FC_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    if (dev >= 0 && dev < 64) cache[dev].store(n, std::memory_order_relaxed);
+  }
+  return n;
+}
+
+// Whether k_fft_cols_pipe takes a column view: one group per slab, all L
+// outputs in order, each tile's 128-byte row segments inside the row pitch
+// (the rows of the work-grid layouts are allocated at full pitch), and its
+// 32-bit in-tile offsets in range.
+template <typename T, typename Tin>
+bool pipe_view_ok(const ColView<T, Tin>& v, int L, int tc, int nout, int offm) {
+  const uint64_t cs = sizeof(cplx<T>);
+  return std::is_same<T, Tin>::value && v.G == 1 && nout == L && offm == 0 &&
+         (int64_t)((v.B + tc - 1) / tc) * tc <= v.inL && (uint64_t)v.inL * cs * 8 < (1ull << 32) &&
+         (uint64_t)v.outL * cs * 64 < (1ull << 32);
+}
+
+// The persistent prefetching column pass (k_fft_cols_pipe): one workgroup
+// per CU (its two tiles and the twiddles take most of the CU's LDS).
+template <typename T, int DIR, int TC, typename A, int Lv>
+void launch_cols_pipe(const ColView<T, T>& v, int tilesB, int64_t nb, const cplx<A>* tw,
+                      hipStream_t s) {
+  static_assert(2 * pipe_lds_bytes<Lv, TC, T>() > 80 * 1024, "one workgroup per CU");
+  const int grid = (int)std::min<int64_t>(nb, current_device_cus());
+  hipLaunchKernelGGL((k_fft_cols_pipe<A, Lv, DIR, TC, T>), dim3((unsigned)grid),
+                     dim3(TC * (Lv / 8)), 0, s, v.in, v.inA, v.inL, v.out, v.outA, v.outL, v.B,
+                     tilesB, (int)nb, tw);
+}
+
 // Register-chained column pass with plan variant var: arithmetic in A
-// (twiddles tw in A), loads Tin, LDS and stores in T.
+// (twiddles tw in A), loads Tin, LDS and stores in T.  Full 512-point passes
+// over FP32 storage (16-column tiles, plan variant 0) run as k_fft_cols_pipe;
+// with FP64 storage its 8-column tiles leave one 8-wave workgroup per CU,
+// which measured no faster than two workgroups of the plain pass.
 template <typename T, int DIR, typename Tin, int TC, typename A = T>
 void launch_cols_rc(int L, const ColView<T, Tin>& v, const cplx<A>* tw, hipStream_t s,
                     int var = 0) {
@@ -93,6 +134,14 @@ void launch_cols_rc(int L, const ColView<T, Tin>& v, const cplx<A>* tw, hipStrea
   const int tilesB = (v.B + TC - 1) / TC;
   const int64_t nb = (int64_t)v.A * v.G * tilesB;
   if (nb > INT32_MAX) fail(FC_ERR_INVALID, "transform too large for one launch");
+  if constexpr (std::is_same<T, Tin>::value && std::is_same<T, float>::value &&
+                TC * sizeof(cplx<T>) == 128) {
+    if (L == 512 && var == 0 && pipe_view_ok(v, L, TC, nout, offm)) {
+      launch_cols_pipe<T, DIR, TC, A, 512>(v, tilesB, nb, tw, s);
+      FC_LAUNCH_CHECK();
+      return;
+    }
+  }
 #define FC_COLS_RC_CASE(Lv)                                                              \
   case Lv:                                                                               \
     if constexpr (FftCfgV<Lv, 1>::fast) {                                                \

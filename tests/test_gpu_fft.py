@@ -7,8 +7,12 @@ from gpu_helpers import fft_c2r, fft_r2c
 
 pytestmark = pytest.mark.gpu
 
+# (4, 512, 64) and (32, 512, 512): 512-point y passes, which run as the
+# persistent prefetching column kernel (the second with several tiles per
+# workgroup).
 SHAPES = [(8, 8, 8), (16, 16, 16), (4, 6, 10), (5, 6, 7), (12, 10, 9), (24, 48, 96),
-          (64, 32, 128), (128, 128, 128), (8, 16, 384), (3, 5, 2), (1, 1, 4)]
+          (64, 32, 128), (128, 128, 128), (8, 16, 384), (3, 5, 2), (1, 1, 4), (4, 512, 64),
+          (32, 512, 512)]
 TOL = {"fp64": 1e-12, "fp32": 3e-6}
 
 
