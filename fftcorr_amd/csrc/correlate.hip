@@ -891,6 +891,23 @@ void launch_periodic_x2(fc_corr* c, const cplx<T>* a, const cplx<T>* b, const Px
                           c->ph[2].as<double2>()};
   const int rs = c->rshape;
   const int offm = (rs / 2) % p.n[0];
+  if constexpr (std::is_same<T, float>::value && MODE == kModeCross) {
+    // Cross mode, 512-point x over FP32 spectra, one correlation: the
+    // persistent prefetching form, when each tile's 64-byte row segments lie
+    // inside the row pitch and its in-tile offsets fit 32 bits.
+    const uint64_t pitch = (uint64_t)p.n[1] * p.ldc * sizeof(cplx<T>);
+    if (p.n[0] == 512 && po.nb == 1 && (int64_t)tilesC * TC <= p.ldc &&
+        pitch * 16 < (1ull << 32)) {
+      const int grid = (int)std::min<int64_t>(nblocks, current_device_cus());
+      hipLaunchKernelGGL((k_periodic_x2_cross_pipe<TI, 512>), dim3((unsigned)grid),
+                         dim3(px_threads<T, 512>()), 0, c->ctx->stream, a, b, p.n[1], p.n2h,
+                         (int64_t)p.ldc, tilesC, p.tw0.as<cplx<T>>(), pi.tw0.as<cplx<TI>>(),
+                         static_cast<T*>(po.pk), kslots(c), static_cast<cplx<TI>*>(po.q1),
+                         (int64_t)pi.ldc, rs, offm, (int)nblocks);
+      FC_LAUNCH_CHECK();
+      return;
+    }
+  }
 #define FC_PX2_CASE(Lv)                                                                       \
   case Lv:                                                                                    \
     hipLaunchKernelGGL((k_periodic_x2<T, TI, Lv, MODE>), dim3((unsigned)nblocks),             \

@@ -666,22 +666,34 @@ __device__ __forceinline__ void lds_dma16(const char* gbase, uint32_t lane_off, 
 
 __device__ __forceinline__ void wait_vm_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-// A tile (L rows of 128 bytes from src, row pitch rowb bytes) into buf:
-// block k of 1 KB holds rows 8k .. 8k + 7, one wave instruction each.
-template <int L, int TC>
-__device__ __forceinline__ void pipe_fetch(const char* src, uint32_t rowb, void* buf) {
-  constexpr int kWaves = TC * (L / 8) / 64;
-  constexpr int kPer = (L / 8) / kWaves;  // blocks per wave
+// NROWS rows of ROWB bytes (16 <= ROWB <= 1024, a power of two) from src
+// (row pitch `pitch` bytes; src uniform) into buf row-major, by the NTHR
+// threads of the workgroup: block k of 1 KB holds rows k * 1024 / ROWB
+// onward, one wave instruction each.
+template <int NROWS, int ROWB, int NTHR>
+__device__ __forceinline__ void lds_fetch_rows(const char* src, uint32_t pitch, void* buf) {
+  constexpr int kRpb = 1024 / ROWB;  // rows per block
+  constexpr int kLpr = ROWB / 16;    // lanes per row
+  constexpr int kBlocks = NROWS / kRpb;
+  constexpr int kWaves = NTHR / 64;
+  static_assert(NROWS % kRpb == 0 && kBlocks % kWaves == 0, "whole blocks per wave");
+  constexpr int kPer = kBlocks / kWaves;
   const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64);
   const int l = threadIdx.x % 64;
-  const uint32_t lane_off = (uint32_t)(l / 8) * rowb + (uint32_t)(l % 8) * 16u;
+  const uint32_t lane_off = (uint32_t)(l / kLpr) * pitch + (uint32_t)(l % kLpr) * 16u;
   const uint32_t lb = lds_u32(buf);
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int blk = w * kPer + j;
-    lds_dma16(src + (uint64_t)(blk * 8) * rowb, lane_off,
+    lds_dma16(src + (uint64_t)(blk * kRpb) * pitch, lane_off,
               __builtin_amdgcn_readfirstlane(lb + (uint32_t)blk * 1024u));
   }
+}
+
+// A column tile (L rows of 128 bytes) into buf.
+template <int L, int TC>
+__device__ __forceinline__ void pipe_fetch(const char* src, uint32_t rowb, void* buf) {
+  lds_fetch_rows<L, 128, TC*(L / 8)>(src, rowb, buf);
 }
 
 template <int L, int TC, typename Ts>

@@ -49,6 +49,91 @@ __device__ __forceinline__ void put_pk(T* pk, const KSlots& ks, int n0, int n1, 
   }
 }
 
+// Combine of the fused x pass: slot s of v holds element e = t + out_m(s) L/E
+// of the transformed tiles (v[0] grid a; v[NT - 1] grid b when interlaced;
+// g the fully transformed grid 2 at the same elements in cross mode).  Writes
+// P (in TI) to u in the inverse pass's input order and Re P (in T) to every
+// compact k slot that reads it.  pyc: the y and z interlacing phases of this
+// column (interlaced mode); ph0: the x phases.
+template <typename T, typename TI, int L, int MODE, int NT, int NG>
+__device__ __forceinline__ void x2_combine(const cplx<T> (&v)[NT][FftCfg<L>::E],
+                                           const cplx<T> (&g)[NG],
+                                           cplx<TI> (&u)[1][FftCfg<L>::E],
+                                           const double2* ph0, double2 pyc, T* pk,
+                                           const KSlots& ks, int n1, int y, int cg, bool colok,
+                                           int t) {
+  using CI = cplx<TI>;
+  constexpr int E = FftCfg<L>::E;
+  constexpr int TCOL = L / E;
+  const int oz = ks.oz();
+  const bool kcol = colok && cg <= oz;
+  const int ox = ks.ks0 / 2, oy = ks.ks1 / 2;
+  int sy0 = slot_a(y, oy), sy1 = slot_b(y, n1, oy);
+  if (sy1 == sy0) sy1 = -1;
+  const uint32_t pstride = (uint32_t)ks.ks1 * (uint32_t)(oz + 1);
+  const uint32_t prow0 = (uint32_t)(sy0 * (oz + 1) + cg), prow1 = (uint32_t)(sy1 * (oz + 1) + cg);
+  {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int e = t + out_m<L>(s) * TCOL;
+      const TI ax = (TI)v[0][s].x, ay = (TI)v[0][s].y;
+      CI p;
+      if (MODE == kModeInterlaced) {
+        const double2 phd = cmul(ph0[e], pyc);
+        const TI px = (TI)phd.x, py = (TI)phd.y;
+        const TI bx = (TI)v[NT - 1][s].x, by = (TI)v[NT - 1][s].y;
+        const TI fx = TI(0.5) * (ax + (bx * px - by * py));
+        const TI fy = TI(0.5) * (ay + (bx * py + by * px));
+        p = CI{fx * fx + fy * fy, TI(0)};
+      } else if (MODE == kModeCross) {
+        const TI gx = (TI)g[s].x, gy = (TI)g[s].y;
+        p = CI{ax * gx + ay * gy, ay * gx - ax * gy};
+      } else {
+        p = CI{ax * ax + ay * ay, TI(0)};
+      }
+      u[0][out_m<L>(s)] = p;
+      if (kcol) {
+        int sx0 = slot_a(e, ox), sx1 = slot_b(e, L, ox);
+        if (sx1 == sx0) sx1 = -1;
+        const T val = (T)p.x;
+        if (sx0 >= 0) {
+          const uint32_t r = (uint32_t)sx0 * pstride;
+          if (sy0 >= 0) pk[r + prow0] = val;
+          if (sy1 >= 0) pk[r + prow1] = val;
+        }
+        if (sx1 >= 0) {
+          const uint32_t r = (uint32_t)sx1 * pstride;
+          if (sy0 >= 0) pk[r + prow0] = val;
+          if (sy1 >= 0) pk[r + prow1] = val;
+        }
+      }
+    }
+  }
+}
+
+// Pruned store of the inverse x pass: q[sx][y][cg] for every sx < rs holds
+// element (sx - off) mod L (offm = off mod L).
+template <typename TI, int L>
+__device__ __forceinline__ void x2_store_q(const cplx<TI> (&u)[1][FftCfg<L>::E], cplx<TI>* q,
+                                           int n1, int y, int64_t ldq, int cg, int rs, int offm,
+                                           int t) {
+  using CI = cplx<TI>;
+  constexpr int E = FftCfg<L>::E;
+  constexpr int TCOL = L / E;
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    int sx = t + out_m<L>(s) * TCOL + offm;
+    if (sx >= L) sx -= L;
+    for (; sx < rs; sx += L)
+      st_stream(reinterpret_cast<CI*>(reinterpret_cast<char*>(q) +
+                                      (((uint32_t)sx * (uint32_t)n1 + (uint32_t)y) *
+                                           (uint32_t)ldq +
+                                       (uint32_t)cg) * (uint32_t)sizeof(CI)),
+                u[0][s]);
+  }
+}
+
 // ---- the fused kernel -------------------------------------------------------
 // Columns per workgroup: 64 bytes of each row (8 FP32 or 4 FP64 complex).
 template <typename T>
@@ -152,68 +237,113 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
   rstages<T, L, -1, NT, 1, true, 0>(v, lds, L * TC, tw, c, t, Lay{});
 
   // Combine: slot s holds element e = t + out_m(s) L/E of both spectra.
-  const int oz = ks.oz();
-  const bool kcol = colok && cg <= oz;
-  const int ox = ks.ks0 / 2, oy = ks.ks1 / 2;
-  int sy0 = slot_a(y, oy), sy1 = slot_b(y, n1, oy);
-  if (sy1 == sy0) sy1 = -1;
-  const uint32_t pstride = (uint32_t)ks.ks1 * (uint32_t)(oz + 1);
-  const uint32_t prow0 = (uint32_t)(sy0 * (oz + 1) + cg), prow1 = (uint32_t)(sy1 * (oz + 1) + cg);
   double2 pyc = make_double2(1.0, 0.0);
   if (MODE == kModeInterlaced) pyc = cmul(ph1[y], ph2[colok ? cg : 0]);
   CI u[1][E];
-  {
-#pragma clang fp contract(off)
-#pragma unroll
-    for (int s = 0; s < E; ++s) {
-      const int e = t + out_m<L>(s) * TCOL;
-      const TI ax = (TI)v[0][s].x, ay = (TI)v[0][s].y;
-      CI p;
-      if (MODE == kModeInterlaced) {
-        const double2 phd = cmul(ph0[e], pyc);
-        const TI px = (TI)phd.x, py = (TI)phd.y;
-        const TI bx = (TI)v[NT - 1][s].x, by = (TI)v[NT - 1][s].y;
-        const TI fx = TI(0.5) * (ax + (bx * px - by * py));
-        const TI fy = TI(0.5) * (ay + (bx * py + by * px));
-        p = CI{fx * fx + fy * fy, TI(0)};
-      } else if (MODE == kModeCross) {
-        const TI gx = (TI)g[s].x, gy = (TI)g[s].y;
-        p = CI{ax * gx + ay * gy, ay * gx - ax * gy};
-      } else {
-        p = CI{ax * ax + ay * ay, TI(0)};
-      }
-      u[0][out_m<L>(s)] = p;
-      if (kcol) {
-        int sx0 = slot_a(e, ox), sx1 = slot_b(e, L, ox);
-        if (sx1 == sx0) sx1 = -1;
-        const T val = (T)p.x;
-        if (sx0 >= 0) {
-          const uint32_t r = (uint32_t)sx0 * pstride;
-          if (sy0 >= 0) pk[r + prow0] = val;
-          if (sy1 >= 0) pk[r + prow1] = val;
-        }
-        if (sx1 >= 0) {
-          const uint32_t r = (uint32_t)sx1 * pstride;
-          if (sy0 >= 0) pk[r + prow0] = val;
-          if (sy1 >= 0) pk[r + prow1] = val;
-        }
-      }
-    }
-  }
+  x2_combine<T, TI, L, MODE, NT, (MODE == kModeCross ? E : 1)>(v, g, u, ph0, pyc, pk, ks, n1, y,
+                                                               cg, colok, t);
   __syncthreads();  // the forward pass's last LDS reads precede the inverse's stores
   rstages<TI, L, +1, 1, 1, true, 0>(u, ldsi, 0, twi, c, t, Lay{});
-  if (colok) {
+  if (colok) x2_store_q<TI, L>(u, q, n1, y, ldq, cg, rs, offm, t);
+}
+
+// ---- persistent cross-mode fused x pass with direct-to-LDS prefetch ---------
+// The kernel above runs two workgroups per CU, each loading its column tiles,
+// transforming and combining them and storing the pruned inverse in turn; its
+// load and compute phases overlap only when the two workgroups happen to be
+// in different phases.  This form of the cross mode (FP32 spectra, 512-point
+// x: the Yamamoto correlator's per-(l, m) pass) runs one 8-wave workgroup per
+// CU over the tiles k G + xcd_tile(b, G) (G = gridDim.x, so each XCD takes a
+// contiguous range of every round and the two 64-byte tiles of a 128-byte
+// line meet in one L2).  While it works on one tile, the next tile's rows of
+// both spectra stream into its second LDS buffer with direct-to-LDS loads;
+// the stage barriers wait for LDS only and the stage twiddles sit in LDS, so
+// no global load queues behind the prefetch.  Arithmetic, combine and stores
+// are those of k_periodic_x2.  At 512^3: 0.410 ms per pass against 0.480; the
+// interlaced (two forward transforms per tile: 0.519 against 0.489) and auto
+// (0.323 against 0.321) modes keep the two-workgroup kernel.
+template <typename TI, int L>
+__global__ void __launch_bounds__(8 * (L / 8))
+    k_periodic_x2_cross_pipe(const cplx<float>* __restrict__ A, const cplx<float>* __restrict__ B,
+                             int n1, int n2h, int64_t ldc, int tilesC,
+                             const cplx<float>* __restrict__ tw, const cplx<TI>* __restrict__ twi,
+                             float* __restrict__ pk, KSlots ks, cplx<TI>* __restrict__ q,
+                             int64_t ldq, int rs, int offm, int ntiles) {
+  using T = float;
+  using C = cplx<T>;
+  using CI = cplx<TI>;
+  constexpr int E = FftCfg<L>::E;
+  constexpr int TC = px_tc<T>();
+  constexpr int TCOL = L / E;
+  constexpr int SL = stage_twiddle_len<L, 0>();
+  constexpr size_t RAW = 2 * (size_t)L * TC * sizeof(C);  // rows of A, then of B
+  constexpr size_t INV = (size_t)L * TC * sizeof(CI);
+  constexpr size_t BUF = RAW > INV ? RAW : INV;
+  using Rad = typename FftCfg<L>::radices;
+  using Lay = LaySwz<TC>;
+  static_assert(E == 8 && TC * sizeof(C) == 64, "64-byte rows of 8 FP32 columns");
+  __shared__ __align__(16) unsigned char bufA[BUF];
+  __shared__ __align__(16) unsigned char bufB[BUF];
+  __shared__ __align__(16) C twf[SL];
+  __shared__ __align__(16) CI twl[SL];
+
+  const int c = threadIdx.x % TC, t = threadIdx.x / TC;
+  const int G = gridDim.x;
+  const uint32_t sLb = (uint32_t)n1 * (uint32_t)ldc * (uint32_t)sizeof(C);  // x pitch
+  auto fetch = [&](int tile, unsigned char* buf) {
+    const int y = tile / tilesC;
+    const int64_t off = (int64_t)y * ldc + (tile - y * tilesC) * TC;
+    lds_fetch_rows<L, 64, TC * TCOL>(reinterpret_cast<const char*>(A + off), sLb, buf);
+    lds_fetch_rows<L, 64, TC * TCOL>(reinterpret_cast<const char*>(B + off), sLb,
+                                     buf + RAW / 2);
+  };
+
+  int tt = xcd_tile(blockIdx.x, G);
+  if (tt >= ntiles) return;
+  fetch(tt, bufA);
+  for (int i = threadIdx.x; i < SL; i += blockDim.x) {
+    twf[i] = tw[L + i];
+    twl[i] = twi[L + i];
+  }
+  wait_vm_all();
+  __syncthreads();
+  if (tt + G < ntiles) fetch(tt + G, bufB);
+
+  // One tile from cur; afterwards cur receives tile tt + 2 G.
+  auto body = [&](unsigned char* cur) {
+    const int y = tt / tilesC;
+    const int cg = (tt - y * tilesC) * TC + c;
+    const bool colok = cg < n2h;
+    const C* rawA = reinterpret_cast<const C*>(cur);
+    const C* rawB = rawA + L * TC;
+    C v[1][E];
+    C g[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      int sx = t + out_m<L>(s) * TCOL + offm;
-      if (sx >= L) sx -= L;
-      for (; sx < rs; sx += L)
-        st_stream(reinterpret_cast<CI*>(reinterpret_cast<char*>(q) +
-                                        (((uint32_t)sx * (uint32_t)n1 + (uint32_t)y) *
-                                             (uint32_t)ldq +
-                                         (uint32_t)cg) * (uint32_t)sizeof(CI)),
-                  u[0][s]);
+      v[0][s] = rawA[(t + s * TCOL) * TC + c];
+      g[s] = rawB[(t + out_m<L>(s) * TCOL) * TC + c];
     }
+    BarLds::sync();  // the raw tiles are in registers: cur takes the exchanges
+    rstages_<T, L, -1, 1, 1, 0, true, 0, 1, Lay, C, 0, BarLds>(
+        Rad{}, v, reinterpret_cast<C*>(cur), 0, twf, c, t, Lay{});
+    CI u[1][E];
+    x2_combine<T, TI, L, kModeCross, 1, E>(v, g, u, nullptr, make_double2(1.0, 0.0), pk, ks, n1,
+                                           y, cg, colok, t);
+    BarLds::sync();  // the forward pass's last LDS reads precede the inverse's stores
+    rstages_<TI, L, +1, 1, 1, 0, true, 0, 1, Lay, CI, 0, BarLds>(
+        Rad{}, u, reinterpret_cast<CI*>(cur), 0, twl, c, t, Lay{});
+    wait_vm_all();   // the prefetch of tile tt + G and earlier stores are done
+    BarLds::sync();  // in every wave; and every wave has read cur for the last time
+    if (tt + 2 * G < ntiles) fetch(tt + 2 * G, cur);
+    if (colok) x2_store_q<TI, L>(u, q, n1, y, ldq, cg, rs, offm, t);
+  };
+  for (;;) {
+    body(bufA);
+    tt += G;
+    if (tt >= ntiles) break;
+    body(bufB);
+    tt += G;
+    if (tt >= ntiles) break;
   }
 }
 

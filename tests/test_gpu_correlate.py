@@ -106,10 +106,12 @@ def test_periodic_interlaced_against_oracle(case, form):
     _compare(c.results(), want, precision)
 
 
+# (512, 8, 8): the cross-mode fused x pass over a 512-point x axis runs as
+# the persistent prefetching kernel for FP32 spectra.
+@pytest.mark.parametrize("shape", [(32, 16, 24), (512, 8, 8)])
 @pytest.mark.parametrize("form", FORMS, ids=["-".join(f) for f in FORMS])
-def test_periodic_cross_and_grid2_fft(form):
+def test_periodic_cross_and_grid2_fft(form, shape):
     precision, algorithm = form
-    shape = (32, 16, 24)
     a = _field(shape, 1)
     b = _field(shape, 2)
     args = (shape, 5.0, 0, 40.0, 5.0, 0.6, 0.05, 4)
@@ -127,7 +129,8 @@ def test_periodic_cross_and_grid2_fft(form):
 
 YAMAMOTO = [((16, 16, 16), 10.0, (-300.0, 50.0, 20.0), 0, 40.0, 10.0, 0.3, 0.05, 2),
             ((24, 20, 16), 6.0, (100.0, -50.0, 400.0), 2, 50.0, 5.0, 0.4, 0.04, 4),
-            ((32, 32, 32), 5.0, (500.0, 500.0, 500.0), 1, 60.0, 5.0, 0.3, 0.03, 4)]
+            ((32, 32, 32), 5.0, (500.0, 500.0, 500.0), 1, 60.0, 5.0, 0.3, 0.03, 4),
+            ((512, 8, 8), 4.0, (-1500.0, 40.0, -20.0), 3, 60.0, 4.0, 0.5, 0.05, 4)]
 
 
 @pytest.mark.parametrize("form", FORMS, ids=["-".join(f) for f in FORMS])
