@@ -127,6 +127,23 @@ def test_periodic_cross_and_grid2_fft(form, shape):
     _compare(c2.results(), want, precision)
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp32"])
+def test_periodic_cross_512_x_several_tiles_per_workgroup(precision):
+    """576 column tiles of the cross-mode x pass (64 rows x 9 tiles of 8
+    kz columns): more than one workgroup per CU can hold, so the persistent
+    kernel prefetches and swaps its buffers between tiles."""
+    shape = (512, 64, 128)
+    a = _field(shape, 21)
+    b = _field(shape, 22)
+    args = (shape, 4.0, 1, 60.0, 4.0, 0.5, 0.05, 4)
+    ref = O.PeriodicCorrelator(*args)
+    ref.fast_hist = True
+    want = ref.cross_correlate(a, b)
+    c = _make(PeriodicCorrelator, args, precision, "fused")
+    c.cross_correlate(a, b)
+    _compare(c.results(), want, precision)
+
+
 YAMAMOTO = [((16, 16, 16), 10.0, (-300.0, 50.0, 20.0), 0, 40.0, 10.0, 0.3, 0.05, 2),
             ((24, 20, 16), 6.0, (100.0, -50.0, 400.0), 2, 50.0, 5.0, 0.4, 0.04, 4),
             ((32, 32, 32), 5.0, (500.0, 500.0, 500.0), 1, 60.0, 5.0, 0.3, 0.03, 4),
