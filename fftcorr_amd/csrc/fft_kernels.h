@@ -861,9 +861,7 @@ __global__ void __launch_bounds__(1024)
   // (error ~ M / TCOL ulp of FP64) rather than one table load per k.
   C w = twN[t];
   const C wstep = twN[TCOL];
-  for (int k = t; k <= M; k += TCOL) {
-    const C zk = to_cplx<T>(lds[lay(c, k == M ? 0 : k)]);
-    const C zm = to_cplx<T>(lds[lay(c, k == 0 ? 0 : M - k)]);
+  auto post = [&](int k, const C& zk, const C& zm) {
     const T ax = zk.x + zm.x, ay = zk.y - zm.y;
     const T bx = zk.x - zm.x, by = zk.y + zm.y;
     if constexpr (sizeof(T) != 8) w = twN[k];
@@ -871,6 +869,27 @@ __global__ void __launch_bounds__(1024)
     const T py = -(w.x * bx - w.y * by);
     st_stream(dst + k, to_cplx<To>(C{T(0.5) * (ax + px), T(0.5) * (ay + py)}));
     if constexpr (sizeof(T) == 8) w = cmul(w, wstep);
+  };
+  if constexpr (TCOL % 8 == 0) {
+    // Thread t's outputs k = t + j TCOL sit at lay(c, t) + j (TCOL + TCOL / 8)
+    // and their mirrors M - k at lay(c, M - t) - j (TCOL + TCOL / 8) (the pad
+    // slot every 8 elements moves by TCOL / 8 per step): constant offsets.
+    // Thread 0 reads Z[0] for both at k = 0 and also writes k = M.
+    constexpr int kStep = TCOL + TCOL / 8;
+    const int bz = lay(c, t), bm = lay(c, M - t);
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const C zk = to_cplx<T>(lds[bz + j * kStep]);
+      const C zm = to_cplx<T>(lds[j == 0 && t == 0 ? bz : bm - j * kStep]);
+      post(t + j * TCOL, zk, zm);
+    }
+    if (t == 0) {
+      const C z0 = to_cplx<T>(lds[bz]);
+      post(M, z0, z0);
+    }
+  } else {
+    for (int k = t; k <= M; k += TCOL)
+      post(k, to_cplx<T>(lds[lay(c, k == M ? 0 : k)]), to_cplx<T>(lds[lay(c, k == 0 ? 0 : M - k)]));
   }
 }
 
