@@ -607,15 +607,17 @@ __attribute__((amdgpu_waves_per_eu(rc_wide_waves<T, To, V>())))
   const uint32_t cin = (uint32_t)c * (uint32_t)sizeof(cplx<Tin>);
   const uint32_t cout = (uint32_t)c * (uint32_t)sizeof(cplx<To>);
   C v[1][E];
+  {
+    // Columns past B read column b0 (discarded), so the E loads are
+    // unconditional and issue back to back before the first use.
+    const uint32_t cinc = colok ? cin : 0u;
+    cplx<Tin> xs[E];
 #pragma unroll
-  for (int s = 0; s < E; ++s) {
-    if (colok) {
-      const cplx<Tin> x = ld_stream(
-          reinterpret_cast<const cplx<Tin>*>(srcb + (cin + (uint32_t)(t + s * TCOL) * inLb)));
-      v[0][s] = C{T(x.x), T(x.y)};
-    } else {
-      v[0][s] = C{T(0), T(0)};
-    }
+    for (int s = 0; s < E; ++s)
+      xs[s] = ld_stream(
+          reinterpret_cast<const cplx<Tin>*>(srcb + (cinc + (uint32_t)(t + s * TCOL) * inLb)));
+#pragma unroll
+    for (int s = 0; s < E; ++s) v[0][s] = colok ? C{T(xs[s].x), T(xs[s].y)} : C{T(0), T(0)};
   }
   rstages<T, L, DIR, 1, 1, true, V>(v, lds, 0, tw, c, t, LaySwz<TC, E>{});
   if (!colok) return;
@@ -833,20 +835,21 @@ __global__ void __launch_bounds__(1024)
   const bool rowok = row < nrows;
   C v[1][E];
   {
+    // Rows past the end read row 0 (discarded), so the E loads are
+    // unconditional and issue back to back before the first use.
     using P = typename PairOf<Tin>::type;
+    const int64_t rowc = rowok ? row : 0;
     const P* src = reinterpret_cast<const P*>(
-        in2 && row >= nrows1 ? in2 + (row - nrows1) * ldi : in + row * ldi);
-    const auto lr = ld.row(row);
+        in2 && rowc >= nrows1 ? in2 + (rowc - nrows1) * ldi : in + rowc * ldi);
+    const auto lr = ld.row(rowc);
+    P pv[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) pv[s] = ld_stream(src + t + s * TCOL);
 #pragma unroll
     for (int s = 0; s < E; ++s) {
       const int e = t + s * TCOL;
-      C z{T(0), T(0)};
-      if (rowok) {
-        const P p = ld_stream(src + e);
-        z.x = T(lr((double)p.x, 2 * e));
-        z.y = T(lr((double)p.y, 2 * e + 1));
-      }
-      v[0][s] = z;
+      const C z{T(lr((double)pv[s].x, 2 * e)), T(lr((double)pv[s].y, 2 * e + 1))};
+      v[0][s] = rowok ? z : C{T(0), T(0)};
     }
   }
   rstages<T, M, -1, 1, 1, true, V>(v, lds, 0, twM, c, t, lay);
@@ -916,21 +919,41 @@ __global__ void __launch_bounds__(1024)
   const int c = threadIdx.x / TCOL, t = threadIdx.x % TCOL;
   const int64_t row = (int64_t)blockIdx.x * tr + c;
   const bool rowok = row < nrows;
-  const C* src = in + row * ldi;
-  for (int k = t; k <= M; k += TCOL) {
-    C x{T(0), T(0)};
-    if (rowok) x = src[k];
-    if (k == 0 || k == M) x.y = T(0);
-    lds[lay(c, k)] = x;
+  // Thread t handles k = t + j TCOL <= M (j <= E, j = E only at t = 0) and
+  // the pairs p = t + j TCOL <= M / 2 (j <= E / 2, j = E / 2 only at t = 0).
+  // Every global load (row values, twiddles) is issued before the first
+  // use, from valid addresses (rows past the end read row 0, discarded).
+  constexpr int NP = E / 2 + 1;
+  const C* src = in + (rowok ? row : 0) * ldi;
+  C xs[E + 1], wps[NP], wqs[NP];
+#pragma unroll
+  for (int j = 0; j <= E; ++j) xs[j] = src[min(t + j * TCOL, M)];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int p = min(t + j * TCOL, M / 2);
+    wps[j] = twN[p];
+    wqs[j] = twN[M - p];
+  }
+#pragma unroll
+  for (int j = 0; j <= E; ++j) {
+    const int k = t + j * TCOL;
+    if (k <= M) {
+      C x = rowok ? xs[j] : C{T(0), T(0)};
+      if (k == 0 || k == M) x.y = T(0);
+      lds[lay(c, k)] = x;
+    }
   }
   __syncthreads();
   // Z[k] = (X[k] + conj X[M-k]) + i W^-k (X[k] - conj X[M-k]) for k < M, in
   // pairs (p, M - p) so the update is in place.
-  for (int p = t; p <= M / 2; p += TCOL) {
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int p = t + j * TCOL;
+    if (p > M / 2) continue;
     const C xp = lds[lay(c, p)];
     const C xq = lds[lay(c, M - p)];
-    const C wp = twN[p];
-    const C wq = twN[M - p];
+    const C wp = wps[j];
+    const C wq = wqs[j];
     {
       const T ax = xp.x + xq.x, ay = xp.y - xq.y;
       const T bx = xp.x - xq.x, by = xp.y + xq.y;
