@@ -52,35 +52,31 @@ __device__ __forceinline__ void put_pk(T* pk, const KSlots& ks, int n0, int n1, 
 // Combine of the fused x pass: slot s of v holds element e = t + out_m(s) L/E
 // of the transformed tiles (v[0] grid a; v[NT - 1] grid b when interlaced;
 // g the fully transformed grid 2 at the same elements in cross mode).  Writes
-// P (in TI) to u in the inverse pass's input order and Re P (in T) to every
-// compact k slot that reads it.  pyc: the y and z interlacing phases of this
-// column (interlaced mode); ph0: the x phases.
+// P (in TI) to u in the inverse pass's input order and Re P (in T) to pkv[s]
+// for x2_store_pk.  pyc: the y and z interlacing phases of this column
+// (interlaced mode); ph0: the x phases, all loaded before the first use.
 template <typename T, typename TI, int L, int MODE, int NT, int NG>
 __device__ __forceinline__ void x2_combine(const cplx<T> (&v)[NT][FftCfg<L>::E],
                                            const cplx<T> (&g)[NG],
                                            cplx<TI> (&u)[1][FftCfg<L>::E],
-                                           const double2* ph0, double2 pyc, T* pk,
-                                           const KSlots& ks, int n1, int y, int cg, bool colok,
-                                           int t) {
+                                           T (&pkv)[FftCfg<L>::E], const double2* ph0,
+                                           double2 pyc, int t) {
   using CI = cplx<TI>;
   constexpr int E = FftCfg<L>::E;
   constexpr int TCOL = L / E;
-  const int oz = ks.oz();
-  const bool kcol = colok && cg <= oz;
-  const int ox = ks.ks0 / 2, oy = ks.ks1 / 2;
-  int sy0 = slot_a(y, oy), sy1 = slot_b(y, n1, oy);
-  if (sy1 == sy0) sy1 = -1;
-  const uint32_t pstride = (uint32_t)ks.ks1 * (uint32_t)(oz + 1);
-  const uint32_t prow0 = (uint32_t)(sy0 * (oz + 1) + cg), prow1 = (uint32_t)(sy1 * (oz + 1) + cg);
+  double2 phv[MODE == kModeInterlaced ? E : 1];
+  if (MODE == kModeInterlaced) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) phv[s] = ph0[t + out_m<L>(s) * TCOL];
+  }
   {
 #pragma clang fp contract(off)
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      const int e = t + out_m<L>(s) * TCOL;
       const TI ax = (TI)v[0][s].x, ay = (TI)v[0][s].y;
       CI p;
       if (MODE == kModeInterlaced) {
-        const double2 phd = cmul(ph0[e], pyc);
+        const double2 phd = cmul(phv[s], pyc);
         const TI px = (TI)phd.x, py = (TI)phd.y;
         const TI bx = (TI)v[NT - 1][s].x, by = (TI)v[NT - 1][s].y;
         const TI fx = TI(0.5) * (ax + (bx * px - by * py));
@@ -93,21 +89,42 @@ __device__ __forceinline__ void x2_combine(const cplx<T> (&v)[NT][FftCfg<L>::E],
         p = CI{ax * ax + ay * ay, TI(0)};
       }
       u[0][out_m<L>(s)] = p;
-      if (kcol) {
-        int sx0 = slot_a(e, ox), sx1 = slot_b(e, L, ox);
-        if (sx1 == sx0) sx1 = -1;
-        const T val = (T)p.x;
-        if (sx0 >= 0) {
-          const uint32_t r = (uint32_t)sx0 * pstride;
-          if (sy0 >= 0) pk[r + prow0] = val;
-          if (sy1 >= 0) pk[r + prow1] = val;
-        }
-        if (sx1 >= 0) {
-          const uint32_t r = (uint32_t)sx1 * pstride;
-          if (sy0 >= 0) pk[r + prow0] = val;
-          if (sy1 >= 0) pk[r + prow1] = val;
-        }
-      }
+      pkv[s] = (T)p.x;
+    }
+  }
+}
+
+// Re P of slot s (pkv, from x2_combine) to every compact k slot that reads
+// element e = t + out_m(s) L/E of row y, column cg.  Issued after the inverse
+// pass, so no load of the pass waits behind these stores.
+template <typename T, int L>
+__device__ __forceinline__ void x2_store_pk(const T (&pkv)[FftCfg<L>::E], T* pk,
+                                            const KSlots& ks, int n1, int y, int cg, bool colok,
+                                            int t) {
+  constexpr int E = FftCfg<L>::E;
+  constexpr int TCOL = L / E;
+  const int oz = ks.oz();
+  if (!(colok && cg <= oz)) return;
+  const int ox = ks.ks0 / 2, oy = ks.ks1 / 2;
+  int sy0 = slot_a(y, oy), sy1 = slot_b(y, n1, oy);
+  if (sy1 == sy0) sy1 = -1;
+  const uint32_t pstride = (uint32_t)ks.ks1 * (uint32_t)(oz + 1);
+  const uint32_t prow0 = (uint32_t)(sy0 * (oz + 1) + cg), prow1 = (uint32_t)(sy1 * (oz + 1) + cg);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const int e = t + out_m<L>(s) * TCOL;
+    int sx0 = slot_a(e, ox), sx1 = slot_b(e, L, ox);
+    if (sx1 == sx0) sx1 = -1;
+    const T val = pkv[s];
+    if (sx0 >= 0) {
+      const uint32_t r = (uint32_t)sx0 * pstride;
+      if (sy0 >= 0) pk[r + prow0] = val;
+      if (sy1 >= 0) pk[r + prow1] = val;
+    }
+    if (sx1 >= 0) {
+      const uint32_t r = (uint32_t)sx1 * pstride;
+      if (sy0 >= 0) pk[r + prow0] = val;
+      if (sy1 >= 0) pk[r + prow1] = val;
     }
   }
 }
@@ -240,10 +257,11 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
   double2 pyc = make_double2(1.0, 0.0);
   if (MODE == kModeInterlaced) pyc = cmul(ph1[y], ph2[colok ? cg : 0]);
   CI u[1][E];
-  x2_combine<T, TI, L, MODE, NT, (MODE == kModeCross ? E : 1)>(v, g, u, ph0, pyc, pk, ks, n1, y,
-                                                               cg, colok, t);
+  T pkv[E];
+  x2_combine<T, TI, L, MODE, NT, (MODE == kModeCross ? E : 1)>(v, g, u, pkv, ph0, pyc, t);
   __syncthreads();  // the forward pass's last LDS reads precede the inverse's stores
   rstages<TI, L, +1, 1, 1, true, 0>(u, ldsi, 0, twi, c, t, Lay{});
+  x2_store_pk<T, L>(pkv, pk, ks, n1, y, cg, colok, t);
   if (colok) x2_store_q<TI, L>(u, q, n1, y, ldq, cg, rs, offm, t);
 }
 
@@ -327,14 +345,15 @@ __global__ void __launch_bounds__(8 * (L / 8))
     rstages_<T, L, -1, 1, 1, 0, true, 0, 1, Lay, C, 0, BarLds>(
         Rad{}, v, reinterpret_cast<C*>(cur), 0, twf, c, t, Lay{});
     CI u[1][E];
-    x2_combine<T, TI, L, kModeCross, 1, E>(v, g, u, nullptr, make_double2(1.0, 0.0), pk, ks, n1,
-                                           y, cg, colok, t);
+    T pkv[E];
+    x2_combine<T, TI, L, kModeCross, 1, E>(v, g, u, pkv, nullptr, make_double2(1.0, 0.0), t);
     BarLds::sync();  // the forward pass's last LDS reads precede the inverse's stores
     rstages_<TI, L, +1, 1, 1, 0, true, 0, 1, Lay, CI, 0, BarLds>(
         Rad{}, u, reinterpret_cast<CI*>(cur), 0, twl, c, t, Lay{});
     wait_vm_all();   // the prefetch of tile tt + G and earlier stores are done
     BarLds::sync();  // in every wave; and every wave has read cur for the last time
     if (tt + 2 * G < ntiles) fetch(tt + 2 * G, cur);
+    x2_store_pk<T, L>(pkv, pk, ks, n1, y, cg, colok, t);
     if (colok) x2_store_q<TI, L>(u, q, n1, y, ldq, cg, rs, offm, t);
   };
   for (;;) {
