@@ -448,6 +448,18 @@ __device__ __forceinline__ void rstage(cplx<T> (&v)[NT][(FftCfgV<L, V>::E)], CL*
   constexpr int TCOL = L / E;
   constexpr int NB = E / R;
   constexpr int LR = L / R;
+  // The stage's twiddles are loaded first, so their latency overlaps the
+  // LDS reads and the barrier below (FP64: w only; FP32: w^1 .. w^(R-1)).
+  constexpr int NTW = NS > 1 ? (sizeof(T) == 8 ? 1 : R - 1) : 1;
+  C twv[NB][NTW];
+  if (NS > 1) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const C* wk = tws + OFF + ((t + b * TCOL) % NS) * R;
+#pragma unroll
+      for (int r = 0; r < NTW; ++r) twv[b][r] = wk[r + 1];
+    }
+  }
   if (!FIRST) {
 #pragma unroll
     for (int q = Q0; q < Q0 + NQ; ++q)
@@ -463,12 +475,11 @@ __device__ __forceinline__ void rstage(cplx<T> (&v)[NT][(FftCfgV<L, V>::E)], CL*
     const int j = t + b * TCOL;
     const int k = j % NS;
     if (NS > 1) {
-      const C* wk = tws + OFF + k * R;
       if constexpr (sizeof(T) == 8) {
         // FP64: entry r is w^r, w = entry 1; the powers are formed in
         // registers (error ~ r ulp of FP64), which keeps one twiddle live
         // instead of R - 1.
-        C w1 = wk[1];
+        C w1 = twv[b][0];
         if (DIR > 0) w1.y = -w1.y;
         C w = w1;
 #pragma unroll
@@ -480,7 +491,7 @@ __device__ __forceinline__ void rstage(cplx<T> (&v)[NT][(FftCfgV<L, V>::E)], CL*
       } else {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          C w = wk[r];
+          C w = twv[b][r - 1];
           if (DIR > 0) w.y = -w.y;
 #pragma unroll
           for (int q = Q0; q < Q0 + NQ; ++q) v[q][b * R + r] = cmul(v[q][b * R + r], w);
