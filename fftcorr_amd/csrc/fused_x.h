@@ -94,6 +94,14 @@ __device__ __forceinline__ void x2_combine(const cplx<T> (&v)[NT][FftCfg<L>::E],
   }
 }
 
+// A uniform value the optimiser cannot see through: offsets built as
+// base + k * step from it stay additions (folding them back into
+// (t + k) * stride would cost a quarter-rate integer multiply per element).
+__device__ __forceinline__ uint32_t opaque_uniform(uint32_t x) {
+  asm volatile("" : "+s"(x));
+  return x;
+}
+
 // Re P of slot s (pkv, from x2_combine) to every compact k slot that reads
 // element e = t + out_m(s) L/E of row y, column cg.  Issued after the inverse
 // pass, so no load of the pass waits behind these stores.
@@ -110,19 +118,26 @@ __device__ __forceinline__ void x2_store_pk(const T (&pkv)[FftCfg<L>::E], T* pk,
   if (sy1 == sy0) sy1 = -1;
   const uint32_t pstride = (uint32_t)ks.ks1 * (uint32_t)(oz + 1);
   const uint32_t prow0 = (uint32_t)(sy0 * (oz + 1) + cg), prow1 = (uint32_t)(sy1 * (oz + 1) + cg);
+  // Slot rows (e + ox) pstride and (e - L + ox) pstride from e pstride =
+  // t pstride + out_m(s) TCOL pstride: one multiply per thread.
+  const uint32_t tp = (uint32_t)t * pstride;
+  const uint32_t step = opaque_uniform((uint32_t)TCOL * pstride);
+  const uint32_t r0 = opaque_uniform((uint32_t)ox * pstride);
+  const uint32_t r1 = opaque_uniform((uint32_t)(ox - L) * pstride);
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     const int e = t + out_m<L>(s) * TCOL;
     int sx0 = slot_a(e, ox), sx1 = slot_b(e, L, ox);
     if (sx1 == sx0) sx1 = -1;
     const T val = pkv[s];
+    const uint32_t ep = tp + (uint32_t)out_m<L>(s) * step;
     if (sx0 >= 0) {
-      const uint32_t r = (uint32_t)sx0 * pstride;
+      const uint32_t r = ep + r0;
       if (sy0 >= 0) pk[r + prow0] = val;
       if (sy1 >= 0) pk[r + prow1] = val;
     }
     if (sx1 >= 0) {
-      const uint32_t r = (uint32_t)sx1 * pstride;
+      const uint32_t r = ep + r1;
       if (sy0 >= 0) pk[r + prow0] = val;
       if (sy1 >= 0) pk[r + prow1] = val;
     }
@@ -138,16 +153,25 @@ __device__ __forceinline__ void x2_store_q(const cplx<TI> (&u)[1][FftCfg<L>::E],
   using CI = cplx<TI>;
   constexpr int E = FftCfg<L>::E;
   constexpr int TCOL = L / E;
+  // Byte offset sx A + (y ldq + cg) |CI| with A = n1 ldq |CI|: one multiply
+  // per thread, then uniform steps (32-bit arithmetic throughout).
+  const uint32_t A = (uint32_t)n1 * (uint32_t)ldq * (uint32_t)sizeof(CI);
+  int base = t + offm;
+  if (base >= L) base -= L;
+  const uint32_t o0 =
+      (uint32_t)base * A + ((uint32_t)y * (uint32_t)ldq + (uint32_t)cg) * (uint32_t)sizeof(CI);
+  char* qb = reinterpret_cast<char*>(q);
+  const uint32_t stepA = opaque_uniform((uint32_t)TCOL * A);
+  const uint32_t wrapA = opaque_uniform((uint32_t)L * A);
 #pragma unroll
   for (int s = 0; s < E; ++s) {
-    int sx = t + out_m<L>(s) * TCOL + offm;
-    if (sx >= L) sx -= L;
-    for (; sx < rs; sx += L)
-      st_stream(reinterpret_cast<CI*>(reinterpret_cast<char*>(q) +
-                                      (((uint32_t)sx * (uint32_t)n1 + (uint32_t)y) *
-                                           (uint32_t)ldq +
-                                       (uint32_t)cg) * (uint32_t)sizeof(CI)),
-                u[0][s]);
+    int sx = base + out_m<L>(s) * TCOL;
+    uint32_t off = o0 + (uint32_t)out_m<L>(s) * stepA;
+    if (sx >= L) {
+      sx -= L;
+      off -= wrapA;
+    }
+    for (; sx < rs; sx += L, off += wrapA) st_stream(reinterpret_cast<CI*>(qb + off), u[0][s]);
   }
 }
 
@@ -230,7 +254,12 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
   // Byte offsets stay below 4 GB (checked by the launcher): 32-bit address
   // arithmetic on a uniform base.
   const uint32_t sL = (uint32_t)n1 * (uint32_t)ldc * (uint32_t)sizeof(C);
-  const uint32_t base = ((uint32_t)y * (uint32_t)ldc + (uint32_t)cg) * (uint32_t)sizeof(C);
+  // Columns past n2h read column c0 (discarded): the loads are unconditional
+  // and issue back to back; row offsets are base + s TCOL sL.
+  const uint32_t base =
+      ((uint32_t)y * (uint32_t)ldc + (uint32_t)(colok ? cg : c0)) * (uint32_t)sizeof(C) +
+      (uint32_t)t * sL;
+  const uint32_t stepL = opaque_uniform((uint32_t)TCOL * sL);
   const char* Ab = reinterpret_cast<const char*>(A);
   const char* Bb = reinterpret_cast<const char*>(B);
 
@@ -238,17 +267,20 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
   C g[MODE == kModeCross ? E : 1];
 #pragma unroll
   for (int s = 0; s < E; ++s) {
-    const uint32_t o = base + (uint32_t)(t + s * TCOL) * sL;
+    const uint32_t o = base + (uint32_t)s * stepL;
     // Plain loads: the two tiles of a 128-byte line (64 bytes each) are read
     // by neighbouring workgroups of one XCD, the second from its L2.
-    v[0][s] = colok ? *reinterpret_cast<const C*>(Ab + o) : C{T(0), T(0)};
-    if (NT == 2) v[NT - 1][s] = colok ? *reinterpret_cast<const C*>(Bb + o) : C{T(0), T(0)};
+    v[0][s] = *reinterpret_cast<const C*>(Ab + o);
+    if (NT == 2) v[NT - 1][s] = *reinterpret_cast<const C*>(Bb + o);
+    if (MODE == kModeCross)
+      g[s] = *reinterpret_cast<const C*>(Bb + base + (uint32_t)out_m<L>(s) * stepL);
   }
-  if (MODE == kModeCross) {
+  if (!colok) {
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      const uint32_t o = base + (uint32_t)(t + out_m<L>(s) * TCOL) * sL;
-      g[s] = colok ? *reinterpret_cast<const C*>(Bb + o) : C{T(0), T(0)};
+      v[0][s] = C{T(0), T(0)};
+      if (NT == 2) v[NT - 1][s] = C{T(0), T(0)};
+      if (MODE == kModeCross) g[s] = C{T(0), T(0)};
     }
   }
   rstages<T, L, -1, NT, 1, true, 0>(v, lds, L * TC, tw, c, t, Lay{});
