@@ -453,11 +453,14 @@ __device__ __forceinline__ void rstage(cplx<T> (&v)[NT][(FftCfgV<L, V>::E)], CL*
   constexpr int NTW = NS > 1 ? (sizeof(T) == 8 ? 1 : R - 1) : 1;
   C twv[NB][NTW];
   if (NS > 1) {
+    // 32-bit byte offsets from the table's uniform base.
+    const char* twb = reinterpret_cast<const char*>(tws);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const C* wk = tws + OFF + ((t + b * TCOL) % NS) * R;
+      const uint32_t k0 = (uint32_t)(OFF + ((t + b * TCOL) % NS) * R);
 #pragma unroll
-      for (int r = 0; r < NTW; ++r) twv[b][r] = wk[r + 1];
+      for (int r = 0; r < NTW; ++r)
+        twv[b][r] = *reinterpret_cast<const C*>(twb + (k0 + (uint32_t)(r + 1)) * (uint32_t)sizeof(C));
     }
   }
   if (!FIRST) {

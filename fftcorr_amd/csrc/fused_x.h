@@ -66,8 +66,11 @@ __device__ __forceinline__ void x2_combine(const cplx<T> (&v)[NT][FftCfg<L>::E],
   constexpr int TCOL = L / E;
   double2 phv[MODE == kModeInterlaced ? E : 1];
   if (MODE == kModeInterlaced) {
+    const char* phb = reinterpret_cast<const char*>(ph0);
 #pragma unroll
-    for (int s = 0; s < E; ++s) phv[s] = ph0[t + out_m<L>(s) * TCOL];
+    for (int s = 0; s < E; ++s)
+      phv[s] = *reinterpret_cast<const double2*>(
+          phb + (uint32_t)(t + out_m<L>(s) * TCOL) * (uint32_t)sizeof(double2));
   }
   {
 #pragma clang fp contract(off)
@@ -116,14 +119,20 @@ __device__ __forceinline__ void x2_store_pk(const T (&pkv)[FftCfg<L>::E], T* pk,
   const int ox = ks.ks0 / 2, oy = ks.ks1 / 2;
   int sy0 = slot_a(y, oy), sy1 = slot_b(y, n1, oy);
   if (sy1 == sy0) sy1 = -1;
-  const uint32_t pstride = (uint32_t)ks.ks1 * (uint32_t)(oz + 1);
-  const uint32_t prow0 = (uint32_t)(sy0 * (oz + 1) + cg), prow1 = (uint32_t)(sy1 * (oz + 1) + cg);
+  // Byte offsets (below 4 GB: the launcher checks the compact array's size)
+  // from the uniform base, so the stores take a 32-bit offset each.
+  constexpr uint32_t ts = sizeof(T);
+  const uint32_t pstride = (uint32_t)ks.ks1 * (uint32_t)(oz + 1) * ts;
+  const uint32_t prow0 = (uint32_t)(sy0 * (oz + 1) + cg) * ts;
+  const uint32_t prow1 = (uint32_t)(sy1 * (oz + 1) + cg) * ts;
   // Slot rows (e + ox) pstride and (e - L + ox) pstride from e pstride =
   // t pstride + out_m(s) TCOL pstride: one multiply per thread.
   const uint32_t tp = (uint32_t)t * pstride;
   const uint32_t step = opaque_uniform((uint32_t)TCOL * pstride);
   const uint32_t r0 = opaque_uniform((uint32_t)ox * pstride);
   const uint32_t r1 = opaque_uniform((uint32_t)(ox - L) * pstride);
+  char* pkb = reinterpret_cast<char*>(pk);
+  auto put = [&](uint32_t off, T val) { *reinterpret_cast<T*>(pkb + off) = val; };
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     const int e = t + out_m<L>(s) * TCOL;
@@ -133,13 +142,13 @@ __device__ __forceinline__ void x2_store_pk(const T (&pkv)[FftCfg<L>::E], T* pk,
     const uint32_t ep = tp + (uint32_t)out_m<L>(s) * step;
     if (sx0 >= 0) {
       const uint32_t r = ep + r0;
-      if (sy0 >= 0) pk[r + prow0] = val;
-      if (sy1 >= 0) pk[r + prow1] = val;
+      if (sy0 >= 0) put(r + prow0, val);
+      if (sy1 >= 0) put(r + prow1, val);
     }
     if (sx1 >= 0) {
       const uint32_t r = ep + r1;
-      if (sy0 >= 0) pk[r + prow0] = val;
-      if (sy1 >= 0) pk[r + prow1] = val;
+      if (sy0 >= 0) put(r + prow0, val);
+      if (sy1 >= 0) put(r + prow1, val);
     }
   }
 }
