@@ -179,13 +179,8 @@ std::shared_ptr<FftPlan> build_plan(int device, const int shape[3], int prec) {
     p->var[ax] = fft_var_ok(shape[ax], v) ? v : 0;
   }
   if (p->fast[2]) {
-    // Default: radix-8 plans, except 256-point rows of FP32 storage, whose
-    // radix-16 plan (16 values per thread, four rows per wave) the
-    // autotuner picks for the mixed form at 512^3 (z 0.381 -> 0.364 ms).
-    const int M = shape[2] / 2;
-    const int dflt = (M == 256 && cs == sizeof(float2)) ? 1 : 0;
-    const int v = tile_for(M, 4, dflt);
-    p->var[2] = fft_var_ok(M, v) ? v : 0;
+    const int v = tile_for(shape[2] / 2, 4, 0);
+    p->var[2] = fft_var_ok(shape[2] / 2, v) ? v : 0;
   }
   p->zy_chunk_kib = tile_for(0, 2, kDefaultZyChunkKiB);
   {
