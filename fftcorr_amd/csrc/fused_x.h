@@ -246,6 +246,15 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   CI* ldsi = reinterpret_cast<CI*>(smem_raw);
+  // Stage twiddles of the forward (T) and inverse (TI) plans, one copy per
+  // workgroup in LDS (one table when T and TI agree: the same x length and
+  // precision give the same values).  13.8 KB for 512 points in the mixed
+  // form, so two workgroups still fit a CU beside their 64 KB tiles.
+  constexpr int SL = stage_twiddle_len<L, 0>();
+  constexpr bool kSameTw = std::is_same<T, TI>::value;
+  __shared__ __align__(16) C twf[SL > 0 ? SL : 1];
+  __shared__ __align__(16) CI two[(SL > 0 && !kSameTw) ? SL : 1];
+  const CI* twl = kSameTw ? reinterpret_cast<const CI*>(twf) : two;
 
   const int tile_all = xcd_tile(blockIdx.x, gridDim.x);
   const int bat = tile_all / tiles_per_batch;
@@ -292,7 +301,15 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
       if (MODE == kModeCross) g[s] = C{T(0), T(0)};
     }
   }
-  rstages<T, L, -1, NT, 1, true, 0>(v, lds, L * TC, tw, c, t, Lay{});
+  // The twiddle copies wait for their loads, which come after (and so wait
+  // for) the input loads the first stage needs anyway; the first stage's
+  // closing barrier orders them before any stage reads them.
+  for (int i = threadIdx.x; i < SL; i += blockDim.x) {
+    twf[i] = tw[L + i];
+    if constexpr (!kSameTw) two[i] = twi[L + i];
+  }
+  rstages_<T, L, -1, NT, 1, 0, true, 0, NT, Lay, C, 0, BarWg>(Rad{}, v, lds, L * TC, twf, c, t,
+                                                             Lay{});
 
   // Combine: slot s holds element e = t + out_m(s) L/E of both spectra.
   double2 pyc = make_double2(1.0, 0.0);
@@ -301,7 +318,7 @@ __attribute__((amdgpu_waves_per_eu(px_waves<T>())))
   T pkv[E];
   x2_combine<T, TI, L, MODE, NT, (MODE == kModeCross ? E : 1)>(v, g, u, pkv, ph0, pyc, t);
   __syncthreads();  // the forward pass's last LDS reads precede the inverse's stores
-  rstages<TI, L, +1, 1, 1, true, 0>(u, ldsi, 0, twi, c, t, Lay{});
+  rstages_<TI, L, +1, 1, 1, 0, true, 0, 1, Lay, CI, 0, BarWg>(Rad{}, u, ldsi, 0, twl, c, t, Lay{});
   x2_store_pk<T, L>(pkv, pk, ks, n1, y, cg, colok, t);
   if (colok) x2_store_q<TI, L>(u, q, n1, y, ldq, cg, rs, offm, t);
 }
