@@ -54,7 +54,8 @@ __device__ __forceinline__ void put_pk(T* pk, const KSlots& ks, int n0, int n1, 
 // g the fully transformed grid 2 at the same elements in cross mode).  Writes
 // P (in TI) to u in the inverse pass's input order and Re P (in T) to pkv[s]
 // for x2_store_pk.  pyc: the y and z interlacing phases of this column
-// (interlaced mode); ph0: the x phases, all loaded before the first use.
+// (interlaced mode); ph0: the x phases exp(i pi f / L) of the signed
+// frequencies f, loaded before the first use.
 template <typename T, typename TI, int L, int MODE, int NT, int NG>
 __device__ __forceinline__ void x2_combine(const cplx<T> (&v)[NT][FftCfg<L>::E],
                                            const cplx<T> (&g)[NG],
@@ -64,13 +65,17 @@ __device__ __forceinline__ void x2_combine(const cplx<T> (&v)[NT][FftCfg<L>::E],
   using CI = cplx<TI>;
   constexpr int E = FftCfg<L>::E;
   constexpr int TCOL = L / E;
-  double2 phv[MODE == kModeInterlaced ? E : 1];
+  // The x phase of element e = t + m L/E is ph0[t] ph0[m L/E]: with
+  // ph0[a] = exp(i pi f(a) / L) for the signed frequency f, f(e) = t + f(m L/E)
+  // because L/2 is a multiple of L/E and t < L/E.  ph0[t] (times the column's
+  // y and z phases) is one load per thread; the ph0[m L/E] are the same for
+  // every lane (scalar loads).
+  double2 pt = make_double2(1.0, 0.0);
+  double2 pm[MODE == kModeInterlaced ? E : 1];
   if (MODE == kModeInterlaced) {
-    const char* phb = reinterpret_cast<const char*>(ph0);
+    pt = cmul(ph0[t], pyc);
 #pragma unroll
-    for (int s = 0; s < E; ++s)
-      phv[s] = *reinterpret_cast<const double2*>(
-          phb + (uint32_t)(t + out_m<L>(s) * TCOL) * (uint32_t)sizeof(double2));
+    for (int s = 0; s < E; ++s) pm[s] = ph0[out_m<L>(s) * TCOL];
   }
   {
 #pragma clang fp contract(off)
@@ -79,7 +84,7 @@ __device__ __forceinline__ void x2_combine(const cplx<T> (&v)[NT][FftCfg<L>::E],
       const TI ax = (TI)v[0][s].x, ay = (TI)v[0][s].y;
       CI p;
       if (MODE == kModeInterlaced) {
-        const double2 phd = cmul(phv[s], pyc);
+        const double2 phd = cmul(pt, pm[s]);
         const TI px = (TI)phd.x, py = (TI)phd.y;
         const TI bx = (TI)v[NT - 1][s].x, by = (TI)v[NT - 1][s].y;
         const TI fx = TI(0.5) * (ax + (bx * px - by * py));
