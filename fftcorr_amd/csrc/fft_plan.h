@@ -274,10 +274,11 @@ void launch_rows_r2c(const FftPlan& p, const Tin* in, int64_t ldi, cplx<T>* out,
     const cplx<A>* twMa = pt.tw2M.as<cplx<A>>();
     const cplx<A>* twNa = pt.tw2N.as<cplx<A>>();
     const int var = fft_var_ok(M, p.var[2]) ? p.var[2] : 0;
-    // Wider arithmetic: at most one wave per workgroup (2 rows of M = 256):
-    // small independent workgroups wait at no barrier of other waves
-    // (measured 0.231 -> 0.226 ms for the 512^3 z pass against 8 rows).
-    const int cap = std::max(1, 64 / (M / (var == 1 ? 16 : 8)));
+    // Wider arithmetic: at most two waves per workgroup (4 rows of M = 256):
+    // small workgroups wait at few barriers of other waves (the 512^3 mixed
+    // z pass: 0.375 ms with two waves against 0.384 with one; four waves
+    // were slower than one in an earlier form of the kernel).
+    const int cap = std::max(1, 128 / (M / (var == 1 ? 16 : 8)));
     int trc = std::is_same<T, A>::value ? tr : (tr < cap ? tr : cap);
     while (trc > 1 && (size_t)trc * row_pad_len(M) * cs > kLdsBudget) trc /= 2;
     auto go = [&](auto vtag, auto mtag) {
